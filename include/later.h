@@ -1,0 +1,175 @@
+/*
+ * later.h -- C-ABI of the MI355X-native recursive Gram-Schmidt QR.
+ *
+ * Drop-in for the reference's QR entry point
+ *   void later_rgsqrf(cudaCtxt ctxt, int m, int n, float* A, int lda,
+ *                     float* R, int ldr, float* work, int lwork,
+ *                     __half* hwork, int lhwork);          (include/LATER.h:39)
+ * with the same argument order and meaning: column-major FP32 A (m x n, lda)
+ * overwritten by Q, FP32 R (n x n, ldr) receiving the upper-triangular factor,
+ * caller-owned device workspaces `work` (FP32, lwork elements) and `hwork`
+ * (16-bit, lhwork elements).  Work is enqueued on ctxt.stream and the call
+ * returns without synchronising, as the reference enqueues on its cuBLAS
+ * handle's stream (QR/later_rgsqrf.cu:62-79, test/test_qr.cu:80-84).
+ *
+ * Differences (a strict superset of the reference's behaviour):
+ *   - lda is honoured (the reference forces lda := m, QR/later_rgsqrf.cu:70);
+ *   - any m >= n >= 1 is accepted (the reference needs n = 128*2^k, m >= 256);
+ *   - R's strictly-lower triangle is zeroed (the reference leaves the
+ *     off-diagonal lower blocks untouched, QR/panel.cu:321-323);
+ *   - errors are reported as status codes by later_rgsqrf_ex (the reference
+ *     prints and continues, QR/panel.cu:12-15);
+ *   - R, work and hwork must not alias each other or A.
+ * No function here allocates device memory except the -check helpers, the
+ * vendor-baseline timer and the serving pool, each marked below.
+ *
+ * Plain C: no HIP, torch or C++ types appear in the signatures.  Streams are
+ * hipStream_t values passed as void*.
+ */
+#ifndef LATER_H_
+#define LATER_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Replaces cudaCtxt {cublasHandle_t, cusolverDnHandle_t} (include/LATER.h:19-22).
+ * stream: hipStream_t (NULL = the null stream).  handle: reserved, may be NULL. */
+typedef struct laterCtxt {
+    void *stream;
+    void *handle;
+} laterCtxt;
+
+/* Status codes. */
+enum {
+    LATER_OK = 0,
+    LATER_ERR_SHAPE = 1,      /* m < n, n < 1, lda < m, ldr < n, ... */
+    LATER_ERR_WORKSPACE = 2,  /* lwork / lhwork below the query */
+    LATER_ERR_OPTION = 3,     /* unknown option value */
+    LATER_ERR_DEVICE = 4,     /* HIP runtime error */
+    LATER_ERR_BREAKDOWN = 5,  /* zero or non-finite column norm (rank deficiency, NaN/Inf) */
+    LATER_ERR_ALIAS = 6,      /* R / work / hwork overlap */
+    LATER_ERR_BUDGET = 7      /* serving: request exceeds the HBM budget */
+};
+
+/* GEMM operand precision of the recursion nodes. */
+enum {
+    LATER_GEMM_F32 = 0,   /* FP32 MFMA (the plain-FP32 form) */
+    LATER_GEMM_F16 = 1,   /* FP16 operands, FP32 accumulate: the reference's
+                             GemmEx TENSOR_OP pair (QR/later_rgsqrf.cu:45-56) */
+    LATER_GEMM_BF16 = 2,  /* BF16 operands, FP32 accumulate */
+    LATER_GEMM_F16X3 = 3  /* FP16 hi/lo split of both operands, three MFMA
+                             products, power-of-two column scaling */
+};
+
+/* Panel (leaf) algorithm. */
+enum {
+    LATER_PANEL_MGS = 0,    /* reference form: TSQR tree of block MGS, FP32 sums */
+    LATER_PANEL_MGS64 = 1   /* same tree, FP64 column norms and dot products */
+};
+
+/* Re-orthogonalisation. */
+enum {
+    LATER_REORTH_NONE = 0,  /* single pass (the reference) */
+    LATER_REORTH_NODE = 1,  /* second projection at every node (block CGS2) */
+    LATER_REORTH_PASS = 2   /* second full factorisation of Q, R := R2 * R1 */
+};
+
+typedef struct later_opts {
+    int gemm;         /* LATER_GEMM_* for nodes wider than nmin */
+    int gemm_pass2;   /* LATER_GEMM_* for the nodes of the second pass */
+    int panel;        /* LATER_PANEL_* */
+    int reorth;       /* LATER_REORTH_* */
+    int nmin;         /* nodes of width <= nmin use FP32 products (reference: 128) */
+    int chunk;        /* rows per FP32 accumulation chunk folded into FP64 (0 = off) */
+} later_opts;
+
+/* Defaults reproduce the reference form: FP16 GEMMs above 128 columns, FP32
+ * below, FP32 block-MGS TSQR panel, no re-orthogonalisation. */
+void later_opts_default(later_opts *o);
+/* The accuracy-recovering mixed-precision form. */
+void later_opts_mixed(later_opts *o);
+/* The plain-FP32 form: FP32 MFMA everywhere, reference panel, one pass. */
+void later_opts_fp32(later_opts *o);
+
+/* Workspace query: FP32 elements of `work` and 16-bit elements of `hwork`
+ * that later_rgsqrf_ex needs for (m, n, opts). */
+int later_rgsqrf_workspace(int m, int n, const later_opts *opts, int64_t *lwork, int64_t *lhwork);
+
+/* Reference entry point, same argument list as include/LATER.h:39 (hwork is
+ * the reference's __half*).  Runs the default options.  Returns nothing, as
+ * the reference; a shape or workspace error leaves A and R untouched. */
+void later_rgsqrf(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, int ldr,
+                  float *work, int lwork, uint16_t *hwork, int lhwork);
+
+/* Status-returning form with options.  info (optional) receives per-call
+ * counters. */
+typedef struct later_info {
+    int status;
+    int kernels;          /* kernel launches enqueued */
+    int tree_levels;      /* recursion depth */
+    double flops_ref;     /* 2 n^2 (m - n/3), the reference's formula (test/test_qr.cu:85-86) */
+    double flops_mgs;     /* 2 m n^2 */
+} later_info;
+
+int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, int ldr,
+                    float *work, int64_t lwork, uint16_t *hwork, int64_t lhwork,
+                    const later_opts *opts, later_info *info);
+
+/* Synchronises ctxt.stream and reads the device status word that the last
+ * later_rgsqrf_ex / later_caqr_panel call on this `work` left: bit 0 set
+ * means a zero or non-finite column norm (LATER_ERR_BREAKDOWN); the
+ * affected Q column is zero and its R diagonal entry 0. */
+int later_read_status(laterCtxt ctxt, const float *work, int *device_status);
+
+/* Panel of the reference alone: mgs_caqr_panel_256x32 (QR/panel.cu:65-134)
+ * generalised to w <= 32 columns.  A (m x w) becomes Q, R (w x w). */
+int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, float *R, int ldr,
+                     float *work, int64_t lwork, int fp64_sums);
+
+/* The two node GEMMs alone (QR/later_rgsqrf.cu:45-49 and :52-56):
+ *   later_gemm_proj:   C (n1 x n2, ldc)  = Q^T B,   Q m x n1, B m x n2
+ *   later_gemm_update: B (m x n2, ldb)  -= Q C
+ * prec is LATER_GEMM_*; chunk as in later_opts. */
+int later_gemm_proj(laterCtxt ctxt, int m, int n1, int n2, const float *Q, int ldq,
+                    const float *B, int ldb, float *C, int ldc, int prec, int chunk,
+                    float *work, int64_t lwork);
+int later_gemm_update(laterCtxt ctxt, int m, int n1, int n2, const float *Q, int ldq,
+                      const float *C, int ldc, float *B, int ldb, int prec, float *work,
+                      int64_t lwork);
+
+/* Input generator (replaces util/util.cu:93-109, which uses cuRAND):
+ * Philox4x32-10 keyed by (seed, mat), counter = column-major index i + j*m.
+ * uniform: 2u-1 with u from 24 bits; normal: FP64 Box-Muller. */
+int later_generate_uniform(laterCtxt ctxt, int64_t m, int64_t n, float *A, int64_t lda,
+                           uint32_t seed, uint32_t mat);
+int later_generate_normal(laterCtxt ctxt, int64_t m, int64_t n, float *A, int64_t lda,
+                          uint32_t seed, uint32_t mat);
+
+/* -check metrics with FP64 accumulation (test/test_qr.cu:216-268).  These
+ * allocate a small device buffer and synchronise the stream.
+ *   orth: ||I - Q^T Q||_F (the reference prints this divided by n)
+ *   backward: ||A - Q triu(R)||_F / ||A||_F */
+int later_check_orth(laterCtxt ctxt, int m, int n, const float *Q, int ldq, double *orth);
+int later_check_backward(laterCtxt ctxt, int m, int n, const float *A, int lda, const float *Q,
+                         int ldq, const float *R, int ldr, double *backward);
+
+/* Kernel-class timing with HIP events on the launch stream (instrumentation
+ * for benchmarks; off by default).  Classes: 0 node projection GEMM, 1 node
+ * update GEMM, 2 panel block-MGS kernel, 3 other.  later_prof_end
+ * synchronises and fills 4-entry arrays: launches, total ms, algorithmic
+ * flops and algorithmic HBM bytes.  Host-side, process-wide state. */
+void later_prof_begin(void);
+int later_prof_end(int *launches, double *ms, double *flops, double *bytes);
+
+/* Library version string. */
+const char *later_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LATER_H_ */

@@ -1,0 +1,292 @@
+"""MI355X-native recursive Gram-Schmidt QR (LATER's ``later_rgsqrf``).
+
+Thin Python face of the C-ABI in ``include/later.h``; the compute path is the
+hand-written HIP library ``liblater_amd.so`` built from ``later_amd/csrc``.
+Device memory and streams come from PyTorch-ROCm tensors (plumbing only): every
+matrix is FP32 column-major, i.e. a tensor ``A`` of shape (m, n) with
+``A.stride(0) == 1`` and leading dimension ``A.stride(1)``; ``colmajor(m, n)``
+allocates one.
+
+There is no fallback: if the HIP library is missing or fails to load, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblater_amd.so")
+
+# Status codes (include/later.h)
+OK, ERR_SHAPE, ERR_WORKSPACE, ERR_OPTION, ERR_DEVICE, ERR_BREAKDOWN, ERR_ALIAS, ERR_BUDGET = range(8)
+GEMM_F32, GEMM_F16, GEMM_BF16, GEMM_F16X3 = range(4)
+PANEL_MGS, PANEL_MGS64 = range(2)
+REORTH_NONE, REORTH_NODE, REORTH_PASS = range(3)
+
+_STATUS_NAMES = {
+    ERR_SHAPE: "bad shape", ERR_WORKSPACE: "workspace too small", ERR_OPTION: "bad option",
+    ERR_DEVICE: "HIP runtime error", ERR_BREAKDOWN: "breakdown (zero or non-finite column norm)",
+    ERR_ALIAS: "aliased buffers", ERR_BUDGET: "exceeds HBM budget",
+}
+
+
+class LaterError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        super().__init__(f"{what}: status {code} ({_STATUS_NAMES.get(code, 'unknown')})")
+        self.code = code
+
+
+class laterCtxt(ctypes.Structure):
+    _fields_ = [("stream", ctypes.c_void_p), ("handle", ctypes.c_void_p)]
+
+
+class later_opts(ctypes.Structure):
+    _fields_ = [("gemm", ctypes.c_int), ("gemm_pass2", ctypes.c_int), ("panel", ctypes.c_int),
+                ("reorth", ctypes.c_int), ("nmin", ctypes.c_int), ("chunk", ctypes.c_int)]
+
+    def __repr__(self):
+        return ("later_opts(gemm=%d, gemm_pass2=%d, panel=%d, reorth=%d, nmin=%d, chunk=%d)"
+                % (self.gemm, self.gemm_pass2, self.panel, self.reorth, self.nmin, self.chunk))
+
+
+class later_info(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("kernels", ctypes.c_int), ("tree_levels", ctypes.c_int),
+                ("flops_ref", ctypes.c_double), ("flops_mgs", ctypes.c_double)]
+
+
+_lib = None
+_vp, _i, _i64, _u32, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_uint32, ctypes.c_double
+
+_SIGNATURES = {
+    "later_opts_default": (None, [ctypes.POINTER(later_opts)]),
+    "later_opts_mixed": (None, [ctypes.POINTER(later_opts)]),
+    "later_opts_fp32": (None, [ctypes.POINTER(later_opts)]),
+    "later_rgsqrf_workspace": (_i, [_i, _i, ctypes.POINTER(later_opts), ctypes.POINTER(_i64),
+                                    ctypes.POINTER(_i64)]),
+    "later_rgsqrf": (None, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i]),
+    "later_rgsqrf_ex": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _vp, _i64, _vp, _i64,
+                             ctypes.POINTER(later_opts), ctypes.POINTER(later_info)]),
+    "later_read_status": (_i, [laterCtxt, _vp, ctypes.POINTER(_i)]),
+    "later_caqr_panel": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _vp, _i64, _i]),
+    "later_gemm_proj": (_i, [laterCtxt, _i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _vp, _i64]),
+    "later_gemm_update": (_i, [laterCtxt, _i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _i64]),
+    "later_generate_uniform": (_i, [laterCtxt, _i64, _i64, _vp, _i64, _u32, _u32]),
+    "later_generate_normal": (_i, [laterCtxt, _i64, _i64, _vp, _i64, _u32, _u32]),
+    "later_check_orth": (_i, [laterCtxt, _i, _i, _vp, _i, ctypes.POINTER(_d)]),
+    "later_check_backward": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _vp, _i, ctypes.POINTER(_d)]),
+    "later_version": (ctypes.c_char_p, []),
+    "later_prof_begin": (None, []),
+    "later_prof_end": (_i, [ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python later_amd/build.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+# --------------------------------------------------------------------------
+# Options
+# --------------------------------------------------------------------------
+def opts(form: str = "reference", **over) -> later_opts:
+    """'reference' (FP16 node GEMMs, FP32 MGS panel, one pass), 'fp32'
+    (FP32 MFMA everywhere) or 'mixed' (FP16 hi/lo products, FP64 panel sums,
+    second pass).  Keyword arguments override single fields."""
+    o = later_opts()
+    fn = {"reference": "later_opts_default", "fp32": "later_opts_fp32", "mixed": "later_opts_mixed"}[form]
+    getattr(lib(), fn)(ctypes.byref(o))
+    for k, v in over.items():
+        setattr(o, k, int(v))
+    return o
+
+
+# --------------------------------------------------------------------------
+# Tensors (plumbing)
+# --------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def colmajor(m: int, n: int, device="cuda", dtype=None, ld: int | None = None):
+    """Uninitialised column-major (m x n) FP32 tensor with leading dimension ld."""
+    torch = _torch()
+    ld = m if ld is None else ld
+    buf = torch.empty((n, ld), device=device, dtype=dtype or torch.float32)
+    return buf.t()[:m, :]
+
+
+def _ld(t) -> int:
+    if t.dim() != 2 or t.stride(0) != 1:
+        raise ValueError("expected a column-major 2-D tensor (stride(0) == 1)")
+    return max(t.stride(1), t.shape[0], 1)
+
+
+def ctxt(stream=None) -> laterCtxt:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return laterCtxt(ctypes.c_void_p(s.cuda_stream), None)
+
+
+def _check(rc: int, what: str):
+    if rc != OK:
+        raise LaterError(rc, what)
+
+
+# --------------------------------------------------------------------------
+# QR
+# --------------------------------------------------------------------------
+def workspace_size(m: int, n: int, o: later_opts | None = None) -> int:
+    lw, lh = _i64(), _i64()
+    _check(lib().later_rgsqrf_workspace(m, n, ctypes.byref(o) if o is not None else None,
+                                        ctypes.byref(lw), ctypes.byref(lh)), "later_rgsqrf_workspace")
+    return int(lw.value)
+
+
+def workspace(m: int, n: int, o: later_opts | None = None, device="cuda"):
+    torch = _torch()
+    return torch.empty(max(workspace_size(m, n, o), 1), device=device, dtype=torch.float32)
+
+
+def rgsqrf(A, R=None, o: later_opts | None = None, work=None, stream=None, info: bool = False):
+    """QR of the column-major FP32 tensor A in place: A <- Q.  Returns R (and
+    the later_info when info=True).  Asynchronous on the current stream."""
+    torch = _torch()
+    m, n = A.shape
+    if R is None:
+        R = colmajor(n, n, device=A.device)
+    if work is None:
+        work = workspace(m, n, o, device=A.device)
+    inf = later_info()
+    rc = lib().later_rgsqrf_ex(ctxt(stream), m, n, A.data_ptr(), _ld(A), R.data_ptr(), _ld(R),
+                               work.data_ptr(), work.numel(), None, 0,
+                               ctypes.byref(o) if o is not None else None, ctypes.byref(inf))
+    _check(rc, "later_rgsqrf_ex")
+    return (R, inf) if info else R
+
+
+def rgsqrf_legacy(A, R, work, hwork, stream=None):
+    """The reference-signature entry point (include/LATER.h:39)."""
+    m, n = A.shape
+    lib().later_rgsqrf(ctxt(stream), m, n, A.data_ptr(), _ld(A), R.data_ptr(), _ld(R), work.data_ptr(),
+                       int(work.numel()), hwork.data_ptr() if hwork is not None else None,
+                       int(hwork.numel()) if hwork is not None else 0)
+
+
+def read_status(work, stream=None) -> int:
+    st = _i()
+    _check(lib().later_read_status(ctxt(stream), work.data_ptr(), ctypes.byref(st)), "later_read_status")
+    return int(st.value)
+
+
+def caqr_panel(A, R=None, fp64: bool = False, work=None, stream=None):
+    """mgs_caqr_panel_256x32 (QR/panel.cu:65-134) for an m x w panel, w <= 32."""
+    torch = _torch()
+    m, w = A.shape
+    if R is None:
+        R = colmajor(w, w, device=A.device)
+    if work is None:
+        work = torch.empty(64 + max(1, 40 * m), device=A.device, dtype=torch.float32)
+    _check(lib().later_caqr_panel(ctxt(stream), m, w, A.data_ptr(), _ld(A), R.data_ptr(), _ld(R),
+                                  work.data_ptr(), work.numel(), int(fp64)), "later_caqr_panel")
+    return R
+
+
+def gemm_proj(Q, B, C, prec: int, chunk: int = 0, work=None, stream=None):
+    torch = _torch()
+    m, n1 = Q.shape
+    n2 = B.shape[1]
+    if work is None:
+        work = torch.empty(64 + n2 + 2 * 64 * n1 * n2 + (m // 256 + 2) * n1 * n2 * 2, device=Q.device,
+                           dtype=torch.float32)
+    _check(lib().later_gemm_proj(ctxt(stream), m, n1, n2, Q.data_ptr(), _ld(Q), B.data_ptr(), _ld(B),
+                                 C.data_ptr(), _ld(C), prec, chunk, work.data_ptr(), work.numel()),
+           "later_gemm_proj")
+    return C
+
+
+def gemm_update(Q, C, B, prec: int, work=None, stream=None):
+    torch = _torch()
+    m, n1 = Q.shape
+    n2 = C.shape[1]
+    if work is None:
+        work = torch.empty(n2 + 64, device=Q.device, dtype=torch.float32)
+    _check(lib().later_gemm_update(ctxt(stream), m, n1, n2, Q.data_ptr(), _ld(Q), C.data_ptr(), _ld(C),
+                                   B.data_ptr(), _ld(B), prec, work.data_ptr(), work.numel()),
+           "later_gemm_update")
+    return B
+
+
+# --------------------------------------------------------------------------
+# Inputs and -check metrics
+# --------------------------------------------------------------------------
+def generate_uniform(m: int, n: int, seed: int = 0, mat: int = 0, device="cuda", out=None, stream=None):
+    A = colmajor(m, n, device=device) if out is None else out
+    _check(lib().later_generate_uniform(ctxt(stream), m, n, A.data_ptr(), _ld(A), seed, mat),
+           "later_generate_uniform")
+    return A
+
+
+def generate_normal(m: int, n: int, seed: int = 0, mat: int = 0, device="cuda", out=None, stream=None):
+    A = colmajor(m, n, device=device) if out is None else out
+    _check(lib().later_generate_normal(ctxt(stream), m, n, A.data_ptr(), _ld(A), seed, mat),
+           "later_generate_normal")
+    return A
+
+
+def check_orth(Q, stream=None) -> float:
+    """||I - Q^T Q||_F with FP64 accumulation (test/test_qr.cu:245-268 without the /n)."""
+    m, n = Q.shape
+    out = _d()
+    _check(lib().later_check_orth(ctxt(stream), m, n, Q.data_ptr(), _ld(Q), ctypes.byref(out)),
+           "later_check_orth")
+    return float(out.value)
+
+
+def check_backward(A, Q, R, stream=None) -> float:
+    """||A - Q triu(R)||_F / ||A||_F with FP64 accumulation (test/test_qr.cu:216-228)."""
+    m, n = A.shape
+    out = _d()
+    _check(lib().later_check_backward(ctxt(stream), m, n, A.data_ptr(), _ld(A), Q.data_ptr(), _ld(Q),
+                                      R.data_ptr(), _ld(R), ctypes.byref(out)), "later_check_backward")
+    return float(out.value)
+
+
+PROF_CLASSES = ("gemm_proj", "gemm_update", "panel_mgs", "other")
+
+
+def prof_begin():
+    lib().later_prof_begin()
+
+
+def prof_end() -> dict:
+    """Per kernel class: launches, ms, algorithmic flops and bytes."""
+    n = len(PROF_CLASSES)
+    la, ms, fl, by = (_i * n)(), (_d * n)(), (_d * n)(), (_d * n)()
+    _check(lib().later_prof_end(la, ms, fl, by), "later_prof_end")
+    return {c: {"launches": la[k], "ms": ms[k], "flops": fl[k], "bytes": by[k]} for k, c in enumerate(PROF_CLASSES)}
+
+
+def version() -> str:
+    return lib().later_version().decode()
+
+
+def flops_ref(m: int, n: int) -> float:
+    """The reference's GFLOPS numerator 2 n^2 (m - n/3) (test/test_qr.cu:85-86)."""
+    return 2.0 * n * n * (m - n / 3.0)
+
+
+def flops_mgs(m: int, n: int) -> float:
+    return 2.0 * m * n * n

@@ -1,0 +1,494 @@
+// MFMA GEMMs of the recursion nodes (gfx950).
+//
+// The reference's two tensor-op GemmEx calls per node (QR/later_rgsqrf.cu:45-56),
+// the FP32 Sgemm projections of its 128-column panel (QR/panel.cu:21-61) and
+// the FP32->FP16 staging kernel s2h (util/util.cu:24-32) become one tiled MFMA
+// kernel with the cast fused into the global->LDS path:
+//   projection  C (n1 x n2)  = Q^T B     Q: m x n1, B: m x n2   (K = m, split-K)
+//   update      B (m x n2)  -= Q C       C: n1 x n2             (K = n1)
+// Operands are read as FP32 from HBM and converted in registers to the MFMA
+// input type: FP32 (v_mfma_f32_32x32x2f32), FP16 / BF16 (v_mfma_f32_32x32x16_*)
+// or an FP16 hi/lo pair (hi = fl16(x), lo = fl16((x - hi) 2^11)) that gives
+// three products hi*hi + 2^-11 (hi*lo + lo*hi) accumulated separately.
+// With `chunk` set, the FP32 MFMA accumulators are folded into FP64 registers
+// every `chunk` rows of K, and split-K partials are summed in FP64 in a fixed
+// order, so results are bit-reproducible and independent of the launch.
+#include "common.h"
+#include "gemm.h"
+#include "prof.h"
+
+namespace {
+
+constexpr int BK = 32;
+constexpr int NT = 256;  // 4 waves in a 2 x 2 grid
+
+template <int PREC> struct Lds;
+template <> struct Lds<GEMM_F32> { typedef float T; };
+template <> struct Lds<GEMM_F16> { typedef _Float16 T; };
+template <> struct Lds<GEMM_BF16> { typedef __bf16 T; };
+template <> struct Lds<GEMM_F16X3> { typedef _Float16 T; };
+
+struct GemmArgs {
+    int M, N, K;
+    const float *A; int64_t lda;   // TN: K x M (col i of A = row i of op(A)); NN: M x K
+    const float *B; int64_t ldb;   // K x N
+    float *C; int64_t ldc;         // M x N
+    int a_exp;                     // op(A) scaled by 2^a_exp before the cast
+    const int *b_exp;              // column j of B scaled by 2^b_exp[j] (nullptr: none)
+    int kper;                      // K rows per split (multiple of BK)
+    int chunk_tiles;               // BK tiles per FP64 fold (0: no FP64 accumulation)
+    void *partial;                 // split-K partials (nsplit x M x N), nullptr: write C
+    int update;                    // 0: C = result; 1: C += alpha * result
+    float alpha;
+    int tri;                       // op(A), B upper triangular: skip zero tiles and K ranges
+};
+
+// Loads of one BK-deep slab of an R-row operand tile into registers.
+//   KM (rows contiguous along K): thread owns SEG consecutive k of one row.
+//   !KM (rows contiguous along M): thread owns 4 consecutive rows x KG k.
+template <int R, bool KM> struct Slab {
+    static constexpr int NV = R * BK / NT;
+    static constexpr int SEG = NV;              // KM
+    static constexpr int TPR = BK / SEG;        // KM: threads per row
+    static constexpr int KG = NV / 4;           // !KM: k per thread
+    float v[NV];
+
+    __device__ __forceinline__ void load(const float *P, int64_t ld, int r0, int rmax, int k0, int kmax,
+                                         bool vec)
+    {
+        const int t = threadIdx.x;
+        if (KM) {
+            const int row = r0 + t / TPR;
+            const int k = k0 + (t % TPR) * SEG;
+            const float *p = P + (int64_t)row * ld + k;
+            if (row < rmax) {
+                if (vec && k + SEG <= kmax) {
+#pragma unroll
+                    for (int e = 0; e < SEG; e += 4) {
+                        floatx4 x = *(const floatx4 *)(p + e);
+                        v[e] = x[0]; v[e + 1] = x[1]; v[e + 2] = x[2]; v[e + 3] = x[3];
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < SEG; ++e) v[e] = (k + e < kmax) ? p[e] : 0.0f;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < SEG; ++e) v[e] = 0.0f;
+            }
+        } else {
+            const int rg = r0 + (t % (R / 4)) * 4;
+            const int kb = k0 + (t / (R / 4)) * KG;
+#pragma unroll
+            for (int kk = 0; kk < KG; ++kk) {
+                const int k = kb + kk;
+                const float *p = P + (int64_t)k * ld + rg;
+                if (k < kmax && vec && rg + 4 <= rmax) {
+                    floatx4 x = *(const floatx4 *)p;
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) v[kk * 4 + rr] = x[rr];
+                } else {
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) v[kk * 4 + rr] = (k < kmax && rg + rr < rmax) ? p[rr] : 0.0f;
+                }
+            }
+        }
+    }
+
+    // Row and k (within the tile) of value e.
+    __device__ __forceinline__ int row_of(int e) const
+    {
+        const int t = threadIdx.x;
+        return KM ? t / TPR : (t % (R / 4)) * 4 + (e & 3);
+    }
+    __device__ __forceinline__ int k_of(int e) const
+    {
+        const int t = threadIdx.x;
+        return KM ? (t % TPR) * SEG + e : (t / (R / 4)) * KG + (e >> 2);
+    }
+};
+
+// LDS tile layouts.  16-bit types: [row][BK+8] (k contiguous, 16-byte MFMA
+// fragment reads).  FP32: [k][R+4] (row contiguous, 4-byte fragment reads).
+template <int PREC, int R> struct Tile {
+    typedef typename Lds<PREC>::T T;
+    static constexpr bool F32 = (PREC == GEMM_F32);
+    static constexpr int LD = F32 ? (R + 4) : (BK + 8);
+    static constexpr int SZ = F32 ? BK * LD : R * LD;
+    T hi[SZ];
+    T lo[PREC == GEMM_F16X3 ? SZ : 1];
+
+    __device__ __forceinline__ int idx(int row, int k) const { return F32 ? k * LD + row : row * LD + k; }
+};
+
+template <int PREC, int R, bool KM>
+__device__ __forceinline__ void store_slab(Tile<PREC, R> &t, const Slab<R, KM> &s, float scale)
+{
+    typedef typename Lds<PREC>::T T;
+#pragma unroll
+    for (int e = 0; e < Slab<R, KM>::NV; ++e) {
+        const int row = s.row_of(e), k = s.k_of(e);
+        float x = s.v[e] * scale;
+        if constexpr (PREC == GEMM_F16X3) {
+            _Float16 h = (_Float16)x;
+            _Float16 l = (_Float16)((x - (float)h) * 2048.0f);
+            t.hi[t.idx(row, k)] = h;
+            t.lo[t.idx(row, k)] = l;
+        } else {
+            t.hi[t.idx(row, k)] = (T)x;
+        }
+    }
+}
+
+template <int PREC> struct Frag;
+template <> struct Frag<GEMM_F32> { typedef float T; };
+template <> struct Frag<GEMM_F16> { typedef half8_t T; };
+template <> struct Frag<GEMM_BF16> { typedef bf16x8_t T; };
+template <> struct Frag<GEMM_F16X3> { typedef half8_t T; };
+
+template <int PREC>
+__device__ __forceinline__ floatx16 mfma(typename Frag<PREC>::T a, typename Frag<PREC>::T b, floatx16 c)
+{
+    if constexpr (PREC == GEMM_F32) return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    else if constexpr (PREC == GEMM_BF16) return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+// Fragment of 32 rows starting at `row0` for k-step `ks`.
+// 32x32x16 (16-bit): lane l supplies row l%32, k = 16 ks + 8 (l/32) + 0..7.
+// 32x32x2 (FP32):    lane l supplies row l%32, k = 2 ks + l/32.
+template <int PREC, int R>
+__device__ __forceinline__ typename Frag<PREC>::T frag(const typename Lds<PREC>::T *base, int row0, int ks)
+{
+    const int lane = threadIdx.x & 63;
+    typedef Tile<PREC, R> TT;
+    if constexpr (PREC == GEMM_F32) {
+        return base[(2 * ks + (lane >> 5)) * TT::LD + row0 + (lane & 31)];
+    } else {
+        return *(const typename Frag<PREC>::T *)(base + (row0 + (lane & 31)) * TT::LD + 16 * ks + 8 * (lane >> 5));
+    }
+}
+
+template <int PREC, int BM, int BN, bool AKM, bool CH64>
+__global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
+{
+    constexpr int WM = BM / 2, WN = BN / 2;     // wave tile
+    constexpr int FM = WM / 32, FN = WN / 32;   // 32x32 MFMA blocks per wave
+    constexpr bool X3 = (PREC == GEMM_F16X3);
+    constexpr int KSTEPS = (PREC == GEMM_F32) ? BK / 2 : BK / 16;
+
+    __shared__ Tile<PREC, BM> As;
+    __shared__ Tile<PREC, BN> Bs;
+
+    const int tm = blockIdx.x, tn = blockIdx.y, split = blockIdx.z;
+    const int m0 = tm * BM, n0 = tn * BN;
+    int kbeg = split * g.kper;
+    int kend = min(g.K, kbeg + g.kper);
+    if (g.tri) {
+        // Upper-triangular product: rows >= cols of the tile contribute nothing
+        // below the diagonal, and only k in [m0, n0 + BN) is non-zero.
+        if (m0 > n0 + BN - 1) return;
+        kbeg = max(kbeg, m0);
+        kend = min(kend, n0 + BN);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    const bool avec = ((g.lda & 3) == 0) && ((((uintptr_t)g.A) & 15) == 0);
+    const bool bvec = ((g.ldb & 3) == 0) && ((((uintptr_t)g.B) & 15) == 0);
+
+    // Operand scales (powers of two, so exact).  A has one global scale; a
+    // thread's B values all lie in one column of B (B rows are K-contiguous).
+    Slab<BM, AKM> sa;
+    Slab<BN, true> sb;
+    const float ascale = ldexpf(1.0f, g.a_exp);
+    const int bcol = n0 + sb.row_of(0);
+    const float bscale = (g.b_exp && bcol < g.N) ? ldexpf(1.0f, g.b_exp[bcol]) : 1.0f;
+
+    floatx16 acc[FM][FN], accl[X3 ? FM : 1][X3 ? FN : 1];
+    double acc64[CH64 ? FM : 1][CH64 ? FN : 1][CH64 ? 16 : 1];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+            if constexpr (X3) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) accl[i][j][r] = 0.0f;
+            }
+            if constexpr (CH64) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc64[i][j][r] = 0.0;
+            }
+        }
+
+    auto fold = [&]() {
+        if constexpr (CH64) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        double v = (double)acc[i][j][r];
+                        if constexpr (X3) { v += (double)accl[i][j][r] * 0x1p-11; accl[i][j][r] = 0.0f; }
+                        acc64[i][j][r] += v;
+                        acc[i][j][r] = 0.0f;
+                    }
+        }
+    };
+
+    const int ntiles = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
+    if (ntiles > 0) {
+        sa.load(g.A, g.lda, m0, g.M, kbeg, kend, avec);
+        sb.load(g.B, g.ldb, n0, g.N, kbeg, kend, bvec);
+    }
+    for (int it = 0; it < ntiles; ++it) {
+        store_slab<PREC, BM, AKM>(As, sa, ascale);
+        store_slab<PREC, BN, true>(Bs, sb, bscale);
+        __syncthreads();
+        if (it + 1 < ntiles) {
+            const int k0 = kbeg + (it + 1) * BK;
+            sa.load(g.A, g.lda, m0, g.M, k0, kend, avec);
+            sb.load(g.B, g.ldb, n0, g.N, k0, kend, bvec);
+        }
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            typename Frag<PREC>::T fa[FM], fb[FN];
+            typename Frag<PREC>::T fal[X3 ? FM : 1], fbl[X3 ? FN : 1];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                fa[i] = frag<PREC, BM>(As.hi, wm * WM + 32 * i, ks);
+                if constexpr (X3) fal[i] = frag<PREC, BM>(As.lo, wm * WM + 32 * i, ks);
+            }
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                fb[j] = frag<PREC, BN>(Bs.hi, wn * WN + 32 * j, ks);
+                if constexpr (X3) fbl[j] = frag<PREC, BN>(Bs.lo, wn * WN + 32 * j, ks);
+            }
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    acc[i][j] = mfma<PREC>(fa[i], fb[j], acc[i][j]);
+                    if constexpr (X3) {
+                        accl[i][j] = mfma<PREC>(fa[i], fbl[j], accl[i][j]);
+                        accl[i][j] = mfma<PREC>(fal[i], fb[j], accl[i][j]);
+                    }
+                }
+        }
+        if (CH64 && ((it + 1) % g.chunk_tiles == 0 || it + 1 == ntiles)) fold();
+        __syncthreads();
+    }
+
+    // Epilogue.  32x32 accumulator layout: lane l holds column l%32 and rows
+    // 8 (r/4) + 4 (l/32) + r%4, r = 0..15.
+    const float oscale_a = ldexpf(1.0f, -g.a_exp);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int col = n0 + wn * WN + 32 * j + (lane & 31);
+            if (col >= g.N) continue;
+            const float cs = oscale_a * ((g.b_exp) ? ldexpf(1.0f, -g.b_exp[col]) : 1.0f);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm * WM + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (row >= g.M) continue;
+                double v;
+                if constexpr (CH64) v = acc64[i][j][r];
+                else if constexpr (X3) v = (double)acc[i][j][r] + (double)accl[i][j][r] * 0x1p-11;
+                else v = (double)acc[i][j][r];
+                v *= (double)cs;
+                if (g.partial) {
+                    const int64_t off = ((int64_t)split * g.N + col) * g.M + row;
+                    if constexpr (CH64) ((double *)g.partial)[off] = v;
+                    else ((float *)g.partial)[off] = (float)v;
+                } else if (g.update) {
+                    float *c = g.C + (int64_t)col * g.ldc + row;
+                    if constexpr (CH64) *c = (float)((double)*c + (double)g.alpha * v);
+                    else *c = fmaf(g.alpha, (float)v, *c);
+                } else {
+                    g.C[(int64_t)col * g.ldc + row] = (float)v;
+                }
+            }
+        }
+}
+
+// C = sum over splits (fixed order, FP64) of the partials.
+template <bool F64>
+__global__ void splitk_reduce_kernel(int M, int N, int nsplit, const void *partial, float *C, int64_t ldc)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)M * N) return;
+    const int row = (int)(idx % M), col = (int)(idx / M);
+    double s = 0.0;
+    for (int p = 0; p < nsplit; ++p) {
+        const int64_t off = (int64_t)p * M * N + idx;
+        s += F64 ? ((const double *)partial)[off] : (double)((const float *)partial)[off];
+    }
+    C[(int64_t)col * ldc + row] = (float)s;
+}
+
+// Per-column exponents e_j with max_i |X_ij| 2^e_j in [2^14, 2^15) (0 for a
+// zero column).  One wave per column.
+__global__ void colexp_kernel(int m, int n, const float *X, int64_t ldx, int *e)
+{
+    const int j = blockIdx.x;
+    const float *x = X + (int64_t)j * ldx;
+    float mx = 0.0f;
+    for (int i = threadIdx.x; i < m; i += 64) mx = fmaxf(mx, fabsf(x[i]));
+#pragma unroll
+    for (int mask = 32; mask > 0; mask >>= 1) mx = fmaxf(mx, __shfl_xor(mx, mask, 64));
+    if (threadIdx.x == 0) {
+        int ex = 0;
+        if (mx > 0.0f && mx < INFINITY) {
+            int fe;
+            frexpf(mx, &fe);  // mx in [2^(fe-1), 2^fe)
+            ex = 15 - fe;
+        }
+        e[j] = ex;
+    }
+}
+
+template <int PREC, int BM, int BN, bool AKM, bool CH64>
+void launch_one(hipStream_t s, const GemmArgs &g, dim3 grid)
+{
+    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, AKM, CH64>), grid, dim3(NT), 0, s, g);
+}
+
+template <int PREC, bool AKM, bool CH64>
+void launch_tile(hipStream_t s, const GemmArgs &g, int nsplit, bool small)
+{
+    if (small) {
+        dim3 grid((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
+        launch_one<PREC, 64, 64, AKM, CH64>(s, g, grid);
+    } else {
+        dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
+        launch_one<PREC, 128, 128, AKM, CH64>(s, g, grid);
+    }
+}
+
+template <bool AKM, bool CH64>
+int dispatch(hipStream_t s, int prec, const GemmArgs &g, int nsplit, bool small)
+{
+    switch (prec) {
+    case GEMM_F32: launch_tile<GEMM_F32, AKM, CH64>(s, g, nsplit, small); break;
+    case GEMM_F16: launch_tile<GEMM_F16, AKM, CH64>(s, g, nsplit, small); break;
+    case GEMM_BF16: launch_tile<GEMM_BF16, AKM, CH64>(s, g, nsplit, small); break;
+    case GEMM_F16X3: launch_tile<GEMM_F16X3, AKM, CH64>(s, g, nsplit, small); break;
+    default: return LATER_ERR_OPTION;
+    }
+    HIP_OK(hipGetLastError());
+    return LATER_OK;
+}
+
+} // namespace
+
+// Split plan of the projection, a function of (m, n1, n2, chunk) only, so the
+// summation order (and the result bits) never depends on the caller's
+// workspace: enough workgroups to cover the CUs twice, at least 256 rows of K
+// per split, and a partial buffer of at most 1/8 of the operands' size.
+static void proj_plan(int m, int n1, int n2, int chunk, bool *small, int *nsplit, int *kper)
+{
+    const int64_t tiles128 = ceil_div64(n1, 128) * ceil_div64(n2, 128);
+    *small = (n1 <= 64 || n2 <= 64 || tiles128 < 128);
+    const int bm = *small ? 64 : 128;
+    const int64_t tiles = ceil_div64(n1, bm) * ceil_div64(n2, bm);
+    int64_t ns = ceil_div64(512, tiles);
+    const int64_t maxs = m / 256 > 1 ? m / 256 : 1;
+    if (ns > maxs) ns = maxs;
+    const int64_t per = (int64_t)n1 * n2 * (chunk > 0 ? 2 : 1);
+    const int64_t budget = (int64_t)m * (n1 + n2) / 8 / per;
+    if (ns > budget) ns = budget;
+    if (ns < 1) ns = 1;
+    int64_t kp = ceil_div64(m, ns);
+    const int gran = (chunk > 0) ? chunk : BK;
+    kp = ceil_div64(kp, gran) * gran;
+    *nsplit = (int)ceil_div64(m, kp);
+    *kper = (int)kp;
+}
+
+int64_t gemm_proj_work_floats(int m, int n1, int n2, int chunk)
+{
+    bool small;
+    int ns, kp;
+    proj_plan(m, n1, n2, chunk, &small, &ns, &kp);
+    return ns > 1 ? (int64_t)ns * n1 * n2 * (chunk > 0 ? 2 : 1) : 0;
+}
+
+int gemm_proj(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, int64_t ldq, const float *B,
+              int64_t ldb, float *C, int64_t ldc, int chunk, const int *b_exp, int a_exp, float *work,
+              int64_t wfloats, int *launches)
+{
+    const bool ch64 = chunk > 0;
+    if (ch64 && chunk % BK) return LATER_ERR_OPTION;
+    bool small;
+    int ns, kp;
+    proj_plan(m, n1, n2, chunk, &small, &ns, &kp);
+    if (ns > 1 && wfloats < (int64_t)ns * n1 * n2 * (ch64 ? 2 : 1)) return LATER_ERR_WORKSPACE;
+    GemmArgs g;
+    g.M = n1; g.N = n2; g.K = m;
+    g.A = Q; g.lda = ldq; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+    g.a_exp = a_exp; g.b_exp = b_exp;
+    g.kper = kp;
+    g.chunk_tiles = ch64 ? chunk / BK : 0;
+    g.partial = (ns > 1) ? (void *)work : nullptr;
+    g.update = 0;
+    g.alpha = 1.0f;
+    g.tri = 0;
+    const int tok = prof_start(s);
+    int rc = ch64 ? dispatch<true, true>(s, prec, g, ns, small) : dispatch<true, false>(s, prec, g, ns, small);
+    prof_stop(tok, s, PROF_GEMM_PROJ, 2.0 * m * n1 * n2, 4.0 * ((double)m * (n1 + n2) + (double)n1 * n2));
+    if (rc) return rc;
+    if (launches) ++*launches;
+    if (ns > 1) {
+        const int64_t tot = (int64_t)n1 * n2;
+        dim3 grid((unsigned)ceil_div64(tot, 256));
+        if (ch64) hipLaunchKernelGGL(splitk_reduce_kernel<true>, grid, dim3(256), 0, s, n1, n2, ns, work, C, ldc);
+        else hipLaunchKernelGGL(splitk_reduce_kernel<false>, grid, dim3(256), 0, s, n1, n2, ns, work, C, ldc);
+        HIP_OK(hipGetLastError());
+        if (launches) ++*launches;
+    }
+    return LATER_OK;
+}
+
+int gemm_update(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, int64_t ldq, const float *Cm,
+                int64_t ldc, float *B, int64_t ldb, bool f64, const int *c_exp, int a_exp, int *launches)
+{
+    return gemm_nn(s, prec, m, n2, n1, -1.0f, Q, ldq, Cm, ldc, B, ldb, f64, c_exp, a_exp, false, launches);
+}
+
+int gemm_nn(hipStream_t s, int prec, int M, int N, int K, float alpha, const float *A, int64_t lda,
+            const float *B, int64_t ldb, float *C, int64_t ldc, bool f64, const int *b_exp, int a_exp, bool tri,
+            int *launches)
+{
+    GemmArgs g;
+    g.M = M; g.N = N; g.K = K;
+    g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+    g.a_exp = a_exp; g.b_exp = b_exp;
+    g.kper = K;
+    g.chunk_tiles = f64 ? (int)ceil_div64(K, BK) : 0;
+    g.partial = nullptr;
+    g.update = 1;
+    g.alpha = alpha;
+    g.tri = tri ? 1 : 0;
+    const bool small = (N <= 64);
+    const int tok = prof_start(s);
+    int rc = f64 ? dispatch<false, true>(s, prec, g, 1, small) : dispatch<false, false>(s, prec, g, 1, small);
+    prof_stop(tok, s, PROF_GEMM_UPDATE, tri ? (double)M * N * K / 3.0 : 2.0 * M * N * K,
+              4.0 * ((double)M * K + 2.0 * M * N + (double)K * N));
+    if (rc) return rc;
+    if (launches) ++*launches;
+    return LATER_OK;
+}
+
+int gemm_colexp(hipStream_t s, int m, int n, const float *X, int64_t ldx, int *e, int *launches)
+{
+    if (n <= 0) return LATER_OK;
+    hipLaunchKernelGGL(colexp_kernel, dim3(n), dim3(64), 0, s, m, n, X, ldx, e);
+    HIP_OK(hipGetLastError());
+    if (launches) ++*launches;
+    return LATER_OK;
+}

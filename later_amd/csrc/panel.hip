@@ -1,0 +1,215 @@
+// Tall-skinny panel: TSQR tree of block modified Gram-Schmidt factorisations.
+//
+// Follows the structure of mgs_caqr_panel_256x32 (QR/panel.cu:65-134): every
+// 256-row block is factored by MGS, the blocks' R factors are stacked and the
+// stack is factored recursively, then each block's Q is multiplied by its
+// rows of the stack's Q.  The block kernel is laid out for 64-lane
+// wavefronts instead of the reference's 32x32 thread block of 32-lane warps
+// (QR/panel.cu:246-325): 8 waves, wave v owns columns v, v+8, v+16, v+24 and
+// lane l holds rows l, l+64, ... of them in registers, so the column norm and
+// the projections are single-wave reductions and one barrier per column
+// suffices (the q vector is double-buffered in LDS).
+#include "common.h"
+#include "panel.h"
+#include "prof.h"
+
+namespace {
+
+constexpr int kWaves = 8;
+constexpr int kThreads = kWaves * 64;
+constexpr int kCols = 32;                 // max panel width
+constexpr int kColsPerWave = kCols / kWaves;
+constexpr int kRowsPerLane = 5;           // up to 320 rows per block
+constexpr int kMaxRows = 64 * kRowsPerLane;
+
+template <typename Acc>
+__global__ __launch_bounds__(kThreads) void mgs_block_kernel(int m, int w, float *A, int lda, float *Rout,
+                                                             int ldr, int bh, int nblk, int *status)
+{
+    __shared__ float qs[2][kMaxRows];
+    __shared__ float Rs[kCols][kCols + 1];
+
+    const int b = blockIdx.x;
+    const int r0 = b * bh;
+    // The last block takes every remaining row (a tail shorter than w is merged).
+    const int mm = (b == nblk - 1) ? (m - r0) : bh;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    float *Ab = A + r0;
+
+    float a[kColsPerWave][kRowsPerLane];
+#pragma unroll
+    for (int c = 0; c < kColsPerWave; ++c) {
+        const int j = wave + kWaves * c;
+#pragma unroll
+        for (int t = 0; t < kRowsPerLane; ++t) {
+            const int row = lane + 64 * t;
+            a[c][t] = (j < w && row < mm) ? Ab[row + (int64_t)j * lda] : 0.0f;
+        }
+    }
+    for (int i = threadIdx.x; i < kCols * (kCols + 1); i += kThreads) (&Rs[0][0])[i] = 0.0f;
+    __syncthreads();
+
+    for (int k = 0; k < w; ++k) {
+        const int buf = k & 1;
+        if (wave == (k % kWaves)) {
+            const int slot = k / kWaves;
+            Acc s = 0;
+#pragma unroll
+            for (int t = 0; t < kRowsPerLane; ++t) s += (Acc)a[slot][t] * (Acc)a[slot][t];
+            s = wave_sum(s);
+            Acc nrm = sqrt(s);
+            bool ok = (nrm > (Acc)0) && (nrm < (Acc)INFINITY);
+            if (!ok && lane == 0 && status) atomicOr(status, 1);
+            Acc inv = ok ? (Acc)1 / nrm : (Acc)0;
+#pragma unroll
+            for (int t = 0; t < kRowsPerLane; ++t) {
+                a[slot][t] = (float)((Acc)a[slot][t] * inv);
+                const int row = lane + 64 * t;
+                if (row < kMaxRows) qs[buf][row] = a[slot][t];
+            }
+            if (lane == 0) Rs[k][k] = ok ? (float)nrm : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < kColsPerWave; ++c) {
+            const int j = wave + kWaves * c;
+            if (j > k && j < w) {
+                Acc d = 0;
+#pragma unroll
+                for (int t = 0; t < kRowsPerLane; ++t) d += (Acc)qs[buf][lane + 64 * t] * (Acc)a[c][t];
+                d = wave_sum(d);
+#pragma unroll
+                for (int t = 0; t < kRowsPerLane; ++t)
+                    a[c][t] = (float)((Acc)a[c][t] - (Acc)qs[buf][lane + 64 * t] * d);
+                if (lane == 0) Rs[k][j] = (float)d;
+            }
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int c = 0; c < kColsPerWave; ++c) {
+        const int j = wave + kWaves * c;
+        if (j < w) {
+#pragma unroll
+            for (int t = 0; t < kRowsPerLane; ++t) {
+                const int row = lane + 64 * t;
+                if (row < mm) Ab[row + (int64_t)j * lda] = a[c][t];
+            }
+        }
+    }
+    // R block (w x w, zeros below the diagonal) at rows b*w of Rout.
+    float *Rb = Rout + (int64_t)b * w;
+    for (int i = threadIdx.x; i < w * w; i += kThreads) {
+        const int x = i % w, y = i / w;
+        Rb[x + (int64_t)y * ldr] = (x <= y) ? Rs[x][y] : 0.0f;
+    }
+}
+
+// A_b <- A_b * W_b for every row block b, W_b = rows b*w .. b*w+w-1 of the
+// stacked factor (ld ldw).  Each thread owns whole rows, so the in-place
+// update never reads a value another thread writes (the reference relies on
+// an aliased batched Sgemm here, QR/panel.cu:97-102).
+template <typename Acc>
+__global__ __launch_bounds__(256) void tsqr_apply_kernel(int m, int w, float *A, int lda, const float *W, int ldw,
+                                                         int bh, int nblk)
+{
+    __shared__ float Ws[kCols][kCols];
+    const int b = blockIdx.x;
+    const int r0 = b * bh;
+    const int mm = (b == nblk - 1) ? (m - r0) : bh;
+    const float *Wb = W + (int64_t)b * w;
+    for (int i = threadIdx.x; i < w * w; i += 256) {
+        const int x = i % w, y = i / w;
+        Ws[x][y] = Wb[x + (int64_t)y * ldw];
+    }
+    __syncthreads();
+    float *Ab = A + r0;
+    for (int row = threadIdx.x; row < mm; row += 256) {
+        float v[kCols];
+#pragma unroll
+        for (int k = 0; k < kCols; ++k) v[k] = (k < w) ? Ab[row + (int64_t)k * lda] : 0.0f;
+#pragma unroll 4
+        for (int j = 0; j < w; ++j) {
+            Acc s = 0;
+#pragma unroll
+            for (int k = 0; k < kCols; ++k)
+                if (k < w) s += (Acc)v[k] * (Acc)Ws[k][j];
+            Ab[row + (int64_t)j * lda] = (float)s;
+        }
+    }
+}
+
+struct Blocks {
+    int nblk;
+};
+
+Blocks plan_blocks(int m, int w)
+{
+    const int bh = kPanelBlockRows;
+    int nb = m / bh, r = m % bh;
+    if (nb == 0) return {1};
+    return {nb + ((r >= w) ? 1 : 0)};
+}
+
+} // namespace
+
+int64_t panel_work_floats(int m, int w)
+{
+    if (w < 1) w = 1;
+    int64_t tot = 0;
+    int64_t mm = m;
+    for (;;) {
+        Blocks p = plan_blocks((int)mm, w);
+        if (p.nblk <= 1) break;
+        tot += (int64_t)p.nblk * w * w;
+        mm = (int64_t)p.nblk * w;
+    }
+    return tot;
+}
+
+int panel_tsqr(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr, float *work,
+               int64_t lwork, bool fp64, int *status, int *launches)
+{
+    if (w < 1 || w > kCols || m < w) return LATER_ERR_SHAPE;
+    Blocks p = plan_blocks(m, w);
+    const int bh = kPanelBlockRows;
+    if (p.nblk <= 1) {
+        if (m > kMaxRows) return LATER_ERR_SHAPE;
+        if (fp64)
+            hipLaunchKernelGGL(mgs_block_kernel<double>, dim3(1), dim3(kThreads), 0, s, m, w, A, lda, R, ldr, bh,
+                               1, status);
+        else
+            hipLaunchKernelGGL(mgs_block_kernel<float>, dim3(1), dim3(kThreads), 0, s, m, w, A, lda, R, ldr, bh,
+                               1, status);
+        if (launches) ++*launches;
+        HIP_OK(hipGetLastError());
+        return LATER_OK;
+    }
+    const int ldst = p.nblk * w;
+    const int64_t need = (int64_t)ldst * w;
+    if (lwork < need) return LATER_ERR_WORKSPACE;
+    float *stack = work;
+    const int tok = prof_start(s);
+    if (fp64)
+        hipLaunchKernelGGL(mgs_block_kernel<double>, dim3(p.nblk), dim3(kThreads), 0, s, m, w, A, lda, stack,
+                           ldst, bh, p.nblk, status);
+    else
+        hipLaunchKernelGGL(mgs_block_kernel<float>, dim3(p.nblk), dim3(kThreads), 0, s, m, w, A, lda, stack,
+                           ldst, bh, p.nblk, status);
+    prof_stop(tok, s, PROF_PANEL, 4.0 * m * w * w, 8.0 * m * w + 4.0 * ldst * w);
+    if (launches) ++*launches;
+    HIP_OK(hipGetLastError());
+    int rc = panel_tsqr(s, ldst, w, stack, ldst, R, ldr, work + need, lwork - need, fp64, status, launches);
+    if (rc) return rc;
+    if (fp64)
+        hipLaunchKernelGGL(tsqr_apply_kernel<double>, dim3(p.nblk), dim3(256), 0, s, m, w, A, lda, stack, ldst,
+                           bh, p.nblk);
+    else
+        hipLaunchKernelGGL(tsqr_apply_kernel<float>, dim3(p.nblk), dim3(256), 0, s, m, w, A, lda, stack, ldst,
+                           bh, p.nblk);
+    if (launches) ++*launches;
+    HIP_OK(hipGetLastError());
+    return LATER_OK;
+}

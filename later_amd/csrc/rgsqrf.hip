@@ -1,0 +1,371 @@
+// Recursive Gram-Schmidt QR: host recursion and the C-ABI of include/later.h.
+//
+// The recursion is the reference's (QR/later_rgsqrf.cu:25-60):
+//     [A1 | A2] = [Q1 | Q2] [R11 R12; 0 R22]
+//     Q1 R11 = qr(A1);  R12 = Q1^T A2;  A2 -= Q1 R12;  Q2 R22 = qr(A2)
+// split at n1 = 32 ceil(n/64) (n/2 on the reference domain n = 128*2^k), with
+// nodes of width <= nmin computed with FP32 products as in the reference's
+// 128-column panel (QR/panel.cu:10-63) and leaves of width <= 32 factored by
+// the TSQR panel (QR/panel.cu:65-134).  Everything is enqueued on the caller's
+// stream; nothing synchronises or allocates.
+//
+// Accuracy recovery (absent from the reference):
+//   LATER_REORTH_NODE: a second projection S = Q1^T A2, A2 -= Q1 S, R12 += S
+//                      at every node (block CGS2);
+//   LATER_REORTH_PASS: the whole factorisation is repeated on Q (whose
+//                      condition number is ~1 after the first pass), and
+//                      R := R2 R1 = R1 + (R2 - I) R1.
+#include <string.h>
+
+#include "common.h"
+#include "gemm.h"
+#include "panel.h"
+
+namespace {
+
+constexpr int kHeaderFloats = 64;  // [0] device status word
+constexpr int kQExp = 14;          // |Q| <= 1: Q scaled by 2^14 before 16-bit casts
+
+int split_n(int n) { return 32 * ((n + 63) / 64); }
+
+bool prec_ok(int p) { return p >= LATER_GEMM_F32 && p <= LATER_GEMM_F16X3; }
+bool scaled(int p) { return p == LATER_GEMM_F16X3; }
+
+struct Layout {
+    int64_t header, exps, panel, gemm, snode, r2, tcopy, total;
+};
+
+// Largest split-K partial buffer over the nodes of the recursion.
+void walk_gemm(int m, int n, int nmin, int chunk_big, int64_t *mx)
+{
+    if (n <= 32) return;
+    const int n1 = split_n(n), n2 = n - n1;
+    const int chunk = (n <= nmin) ? 0 : chunk_big;
+    int64_t w = gemm_proj_work_floats(m, n1, n2, chunk);
+    if (w > *mx) *mx = w;
+    walk_gemm(m, n1, nmin, chunk_big, mx);
+    walk_gemm(m, n2, nmin, chunk_big, mx);
+}
+
+Layout layout(int m, int n, const later_opts &o)
+{
+    Layout L;
+    L.header = kHeaderFloats;
+    L.exps = ((int64_t)n + 63) / 64 * 64;
+    L.panel = panel_work_floats(m, 32);
+    int64_t g = 0;
+    walk_gemm(m, n, o.nmin, o.chunk, &g);
+    if (o.reorth == LATER_REORTH_PASS) walk_gemm(m, n, o.nmin, o.chunk, &g);
+    L.gemm = g;
+    const int n1 = split_n(n);
+    L.snode = (o.reorth == LATER_REORTH_NODE && n > 32) ? (int64_t)n1 * (n - n1) : 0;
+    L.r2 = (o.reorth == LATER_REORTH_PASS) ? (int64_t)n * n : 0;
+    L.tcopy = L.r2;
+    L.total = L.header + L.exps + L.panel + L.gemm + L.snode + L.r2 + L.tcopy;
+    return L;
+}
+
+struct Run {
+    hipStream_t s;
+    later_opts o;
+    int *status;
+    int *exps;
+    float *panel; int64_t panel_n;
+    float *gemm; int64_t gemm_n;
+    float *snode;
+    int launches;
+    int depth;
+};
+
+__global__ void zero_lower_kernel(int n, float *R, int64_t ldr)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * n) return;
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    if (i > j) R[i + (int64_t)j * ldr] = 0.0f;
+}
+
+__global__ void fill_kernel(int64_t n, float *x, float v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+__global__ void status_clear_kernel(int *st) { st[threadIdx.x] = 0; }
+
+// C (m x n, ldc) += S (m x n, lds)
+__global__ void add_kernel(int m, int n, const float *S, int64_t lds, float *C, int64_t ldc)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)m * n) return;
+    const int i = (int)(idx % m), j = (int)(idx / m);
+    C[i + (int64_t)j * ldc] += S[i + (int64_t)j * lds];
+}
+
+// Y (n x n, ld n) = X (upper triangle, ldx); diag(Y) -= shift.
+__global__ void copy_upper_kernel(int n, const float *X, int64_t ldx, float *Y, float shift)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * n) return;
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    float v = (i <= j) ? X[i + (int64_t)j * ldx] : 0.0f;
+    if (i == j) v -= shift;
+    Y[idx] = v;
+}
+
+dim3 grid1(int64_t n) { return dim3((unsigned)ceil_div64(n, 256)); }
+
+int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, int prec, int depth)
+{
+    if (depth > r.depth) r.depth = depth;
+    if (n <= 32)
+        return panel_tsqr(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n,
+                          r.o.panel == LATER_PANEL_MGS64, r.status, &r.launches);
+    const int n1 = split_n(n), n2 = n - n1;
+    const bool inner = (n <= r.o.nmin);
+    const int p = inner ? LATER_GEMM_F32 : prec;
+    const int chunk = inner ? 0 : r.o.chunk;
+    const bool f64 = chunk > 0;
+    int rc = qr_rec(r, m, n1, A, lda, R, ldr, prec, depth + 1);
+    if (rc) return rc;
+    float *A2 = A + (int64_t)n1 * lda;
+    float *R12 = R + (int64_t)n1 * ldr;
+    const int aexp = scaled(p) ? kQExp : 0;
+    const int *bexp = nullptr;
+    // R12 = Q1^T A2   (QR/later_rgsqrf.cu:43-49)
+    if (scaled(p)) {
+        if ((rc = gemm_colexp(r.s, m, n2, A2, lda, r.exps, &r.launches))) return rc;
+        bexp = r.exps;
+    }
+    rc = gemm_proj(r.s, p, m, n1, n2, A, lda, A2, lda, R12, ldr, chunk, bexp, aexp, r.gemm, r.gemm_n,
+                   &r.launches);
+    if (rc) return rc;
+    // A2 -= Q1 R12    (QR/later_rgsqrf.cu:51-56)
+    if (scaled(p)) {
+        if ((rc = gemm_colexp(r.s, n1, n2, R12, ldr, r.exps, &r.launches))) return rc;
+    }
+    rc = gemm_update(r.s, p, m, n1, n2, A, lda, R12, ldr, A2, lda, f64, bexp, aexp, &r.launches);
+    if (rc) return rc;
+    if (r.o.reorth == LATER_REORTH_NODE) {
+        // S = Q1^T A2; A2 -= Q1 S; R12 += S
+        float *S = r.snode;
+        if (scaled(p)) {
+            if ((rc = gemm_colexp(r.s, m, n2, A2, lda, r.exps, &r.launches))) return rc;
+        }
+        rc = gemm_proj(r.s, p, m, n1, n2, A, lda, A2, lda, S, n1, chunk, bexp, aexp, r.gemm, r.gemm_n,
+                       &r.launches);
+        if (rc) return rc;
+        if (scaled(p)) {
+            if ((rc = gemm_colexp(r.s, n1, n2, S, n1, r.exps, &r.launches))) return rc;
+        }
+        rc = gemm_update(r.s, p, m, n1, n2, A, lda, S, n1, A2, lda, f64, bexp, aexp, &r.launches);
+        if (rc) return rc;
+        hipLaunchKernelGGL(add_kernel, grid1((int64_t)n1 * n2), dim3(256), 0, r.s, n1, n2, S, (int64_t)n1, R12,
+                           ldr);
+        HIP_OK(hipGetLastError());
+        ++r.launches;
+    }
+    return qr_rec(r, m, n2, A2, lda, R + n1 + (int64_t)n1 * ldr, ldr, prec, depth + 1);
+}
+
+bool overlap(const void *a, int64_t abytes, const void *b, int64_t bbytes)
+{
+    if (!a || !b || abytes <= 0 || bbytes <= 0) return false;
+    const char *x = (const char *)a, *y = (const char *)b;
+    return x < y + bbytes && y < x + abytes;
+}
+
+} // namespace
+
+LATER_API void later_opts_default(later_opts *o)
+{
+    o->gemm = LATER_GEMM_F16;
+    o->gemm_pass2 = LATER_GEMM_F16;
+    o->panel = LATER_PANEL_MGS;
+    o->reorth = LATER_REORTH_NONE;
+    o->nmin = 128;
+    o->chunk = 0;
+}
+
+LATER_API void later_opts_fp32(later_opts *o)
+{
+    later_opts_default(o);
+    o->gemm = LATER_GEMM_F32;
+    o->gemm_pass2 = LATER_GEMM_F32;
+}
+
+LATER_API void later_opts_mixed(later_opts *o)
+{
+    later_opts_default(o);
+    o->gemm = LATER_GEMM_F16X3;
+    o->gemm_pass2 = LATER_GEMM_F16X3;
+    o->panel = LATER_PANEL_MGS64;
+    o->reorth = LATER_REORTH_PASS;
+    o->chunk = 256;
+}
+
+LATER_API int later_rgsqrf_workspace(int m, int n, const later_opts *opts, int64_t *lwork, int64_t *lhwork)
+{
+    later_opts o;
+    if (opts) o = *opts; else later_opts_default(&o);
+    if (m < 1 || n < 1 || m < n) return LATER_ERR_SHAPE;
+    Layout L = layout(m, n, o);
+    if (lwork) *lwork = L.total;
+    if (lhwork) *lhwork = 0;
+    return LATER_OK;
+}
+
+LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, int ldr, float *work,
+                              int64_t lwork, uint16_t *hwork, int64_t lhwork, const later_opts *opts,
+                              later_info *info)
+{
+    later_opts o;
+    if (opts) o = *opts; else later_opts_default(&o);
+    if (info) memset(info, 0, sizeof(*info));
+    int rc = LATER_OK;
+    if (m < 1 || n < 1 || m < n || lda < m || ldr < n || !A || !R) rc = LATER_ERR_SHAPE;
+    else if (!prec_ok(o.gemm) || !prec_ok(o.gemm_pass2) || o.panel < 0 || o.panel > 1 || o.reorth < 0 ||
+             o.reorth > 2 || o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0)
+        rc = LATER_ERR_OPTION;
+    Layout L = layout(m > 0 ? m : 1, n > 0 ? n : 1, o);
+    // Everything lives in `work` when it is large enough; otherwise the
+    // split-K partials may use the 16-bit arena `hwork` (the reference sizes
+    // it at m*n halves, test/test_qr.cu:76, and this library has no FP16
+    // staging buffers to keep there).
+    const bool partial_in_h = work && lwork < L.total && lwork >= L.total - L.gemm && hwork &&
+                              lhwork / 2 >= L.gemm;
+    if (!rc && (!work || (lwork < L.total && !partial_in_h))) rc = LATER_ERR_WORKSPACE;
+    if (!rc) {
+        const int64_t abytes = ((int64_t)(n - 1) * lda + m) * 4, rbytes = ((int64_t)(n - 1) * ldr + n) * 4;
+        const int64_t wbytes = lwork * 4, hbytes = lhwork * 2;
+        if (overlap(A, abytes, R, rbytes) || overlap(A, abytes, work, wbytes) || overlap(R, rbytes, work, wbytes) ||
+            overlap(hwork, hbytes, work, wbytes) || overlap(hwork, hbytes, R, rbytes) ||
+            overlap(hwork, hbytes, A, abytes))
+            rc = LATER_ERR_ALIAS;
+    }
+    if (rc) {
+        if (info) info->status = rc;
+        return rc;
+    }
+    Run r;
+    r.s = ctx_stream(ctxt);
+    r.o = o;
+    float *p = work;
+    r.status = (int *)p; p += L.header;
+    r.exps = (int *)p; p += L.exps;
+    r.panel = p; r.panel_n = L.panel; p += L.panel;
+    if (partial_in_h) {
+        r.gemm = (float *)hwork;
+    } else {
+        r.gemm = p;
+        p += L.gemm;
+    }
+    r.gemm_n = L.gemm;
+    r.snode = p; p += L.snode;
+    float *R2 = p; p += L.r2;
+    float *T = p; p += L.tcopy;
+    r.launches = 0;
+    r.depth = 0;
+
+    hipLaunchKernelGGL(status_clear_kernel, dim3(1), dim3(64), 0, r.s, r.status);
+    hipLaunchKernelGGL(zero_lower_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R, (int64_t)ldr);
+    r.launches += 2;
+    rc = hipGetLastError() == hipSuccess ? LATER_OK : LATER_ERR_DEVICE;
+    if (!rc) rc = qr_rec(r, m, n, A, lda, R, ldr, o.gemm, 0);
+    if (!rc && o.reorth == LATER_REORTH_PASS) {
+        // Second pass on Q, then R = R1 + (R2 - I) R1 (upper-triangular product).
+        hipLaunchKernelGGL(fill_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, (int64_t)n * n, R2, 0.0f);
+        ++r.launches;
+        rc = qr_rec(r, m, n, A, lda, R2, n, o.gemm_pass2, 0);
+        if (!rc) {
+            hipLaunchKernelGGL(copy_upper_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R, (int64_t)ldr, T,
+                               0.0f);
+            hipLaunchKernelGGL(copy_upper_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R2, (int64_t)n, R2,
+                               1.0f);
+            r.launches += 2;
+            const int p2 = o.gemm_pass2;
+            const int *bexp = nullptr;
+            if (scaled(p2)) {
+                rc = gemm_colexp(r.s, n, n, T, n, r.exps, &r.launches);
+                bexp = r.exps;
+            }
+            if (!rc)
+                rc = gemm_nn(r.s, p2, n, n, n, 1.0f, R2, n, T, n, R, ldr, o.chunk > 0, bexp,
+                             scaled(p2) ? kQExp : 0, true, &r.launches);
+        }
+    }
+    if (!rc && hipGetLastError() != hipSuccess) rc = LATER_ERR_DEVICE;
+    if (info) {
+        info->status = rc;
+        info->kernels = r.launches;
+        info->tree_levels = r.depth;
+        info->flops_ref = 2.0 * (double)n * n * ((double)m - n / 3.0);
+        info->flops_mgs = 2.0 * (double)m * n * n;
+    }
+    return rc;
+}
+
+LATER_API void later_rgsqrf(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, int ldr, float *work,
+                            int lwork, uint16_t *hwork, int lhwork)
+{
+    later_rgsqrf_ex(ctxt, m, n, A, lda, R, ldr, work, lwork, hwork, lhwork, nullptr, nullptr);
+}
+
+LATER_API int later_read_status(laterCtxt ctxt, const float *work, int *device_status)
+{
+    if (!work || !device_status) return LATER_ERR_SHAPE;
+    hipStream_t s = ctx_stream(ctxt);
+    HIP_OK(hipMemcpyAsync(device_status, work, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return LATER_OK;
+}
+
+LATER_API int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, float *R, int ldr, float *work,
+                               int64_t lwork, int fp64_sums)
+{
+    if (w < 1 || w > 32 || m < w || lda < m || ldr < w || !A || !R) return LATER_ERR_SHAPE;
+    const int64_t need = kHeaderFloats + panel_work_floats(m, w);
+    if (!work || lwork < need) return LATER_ERR_WORKSPACE;
+    hipStream_t s = ctx_stream(ctxt);
+    hipLaunchKernelGGL(status_clear_kernel, dim3(1), dim3(64), 0, s, (int *)work);
+    return panel_tsqr(s, m, w, A, lda, R, ldr, work + kHeaderFloats, need - kHeaderFloats, fp64_sums != 0,
+                      (int *)work, nullptr);
+}
+
+LATER_API int later_gemm_proj(laterCtxt ctxt, int m, int n1, int n2, const float *Q, int ldq, const float *B,
+                              int ldb, float *C, int ldc, int prec, int chunk, float *work, int64_t lwork)
+{
+    if (m < 1 || n1 < 1 || n2 < 1 || ldq < m || ldb < m || ldc < n1 || !Q || !B || !C) return LATER_ERR_SHAPE;
+    if (!prec_ok(prec) || chunk < 0 || chunk % 32) return LATER_ERR_OPTION;
+    const int64_t exps = ((int64_t)n2 + 63) / 64 * 64;
+    const int64_t need = exps + gemm_proj_work_floats(m, n1, n2, chunk);
+    if (lwork < need || (!work && need > 0)) return LATER_ERR_WORKSPACE;
+    hipStream_t s = ctx_stream(ctxt);
+    const int *bexp = nullptr;
+    if (scaled(prec)) {
+        int rc = gemm_colexp(s, m, n2, B, ldb, (int *)work, nullptr);
+        if (rc) return rc;
+        bexp = (const int *)work;
+    }
+    return gemm_proj(s, prec, m, n1, n2, Q, ldq, B, ldb, C, ldc, chunk, bexp, scaled(prec) ? kQExp : 0,
+                     work + exps, lwork - exps, nullptr);
+}
+
+LATER_API int later_gemm_update(laterCtxt ctxt, int m, int n1, int n2, const float *Q, int ldq, const float *C,
+                                int ldc, float *B, int ldb, int prec, float *work, int64_t lwork)
+{
+    if (m < 1 || n1 < 1 || n2 < 1 || ldq < m || ldb < m || ldc < n1 || !Q || !B || !C) return LATER_ERR_SHAPE;
+    if (!prec_ok(prec)) return LATER_ERR_OPTION;
+    const int64_t exps = ((int64_t)n2 + 63) / 64 * 64;
+    if (scaled(prec) && (lwork < exps || !work)) return LATER_ERR_WORKSPACE;
+    hipStream_t s = ctx_stream(ctxt);
+    const int *bexp = nullptr;
+    if (scaled(prec)) {
+        int rc = gemm_colexp(s, n1, n2, C, ldc, (int *)work, nullptr);
+        if (rc) return rc;
+        bexp = (const int *)work;
+    }
+    return gemm_update(s, prec, m, n1, n2, Q, ldq, C, ldc, B, ldb, false, bexp, scaled(prec) ? kQExp : 0, nullptr);
+}
+
+LATER_API const char *later_version(void) { return "later_amd 0.1 (gfx950)"; }

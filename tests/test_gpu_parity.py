@@ -1,0 +1,276 @@
+"""GPU parity: the HIP library (through the C-ABI) against the oracle.
+
+Tolerances are stated per test.  The oracle's GEMMs accumulate in FP64 while
+the MFMA kernels accumulate in FP32, so floating-point results agree to a few
+units of FP32 rounding scaled by the problem (accumulation length, condition
+number), never bit for bit; integer work (the generator) is bit-exact.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+U32 = 2.0 ** -24
+
+
+def _np(t):
+    return t.detach().cpu().numpy().astype(np.float32, copy=False).copy(order="F")
+
+
+def _dev(later, A):
+    import torch
+    m, n = A.shape
+    T = later.colmajor(m, n)
+    T.copy_(torch.from_numpy(np.asarray(A, dtype=np.float32)))
+    return T
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+# ---------------------------------------------------------------- generator
+@pytest.mark.parametrize("m,n,seed,mat", [(1000, 37, 0, 0), (4096, 3, 7, 11), (1, 1, 0, 5)])
+def test_generate_uniform_bit_exact(later, oracle, m, n, seed, mat):
+    G = later.generate_uniform(m, n, seed, mat)
+    _sync()
+    assert np.array_equal(_np(G), oracle.gen_uniform(m, n, seed, mat))
+
+
+def test_generate_normal_matches(later, oracle):
+    m, n = 4096, 9
+    G = _np(later.generate_normal(m, n, 0, 0))
+    ref = oracle.gen_normal(m, n, 0, 0)
+    # Device and host libm may differ in the last FP64 bit: at most 1 FP32 ulp.
+    ulp = np.abs(G.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 1 and (ulp > 0).mean() < 1e-3
+
+
+# ---------------------------------------------------------------- GEMMs
+def _round(X, prec):
+    import torch
+    t = torch.from_numpy(np.asarray(X, dtype=np.float32))
+    if prec == 1:
+        t = t.half().float()
+    elif prec == 2:
+        t = t.bfloat16().float()
+    return t.numpy().astype(np.float64)
+
+
+PRECS = [(0, "f32"), (1, "f16"), (2, "bf16"), (3, "f16x3")]
+
+
+@pytest.mark.parametrize("m,n1,n2", [(4096, 128, 128), (1000, 100, 36), (16384, 32, 32), (300, 7, 5),
+                                     (2048, 256, 640)])
+@pytest.mark.parametrize("prec,name", PRECS)
+@pytest.mark.parametrize("chunk", [0, 256])
+def test_gemm_proj(later, oracle, m, n1, n2, prec, name, chunk):
+    Q = oracle.gen_uniform(m, n1, 1, 0) * np.float32(0.05)
+    B = oracle.gen_uniform(m, n2, 2, 0)
+    C = later.colmajor(n1, n2)
+    later.gemm_proj(_dev(later, Q), _dev(later, B), C, prec, chunk)
+    _sync()
+    Cg = _np(C).astype(np.float64)
+    if prec == 3:
+        ref = Q.astype(np.float64).T @ B.astype(np.float64)
+        tol = 4e-6
+    else:
+        ref = _round(Q, prec).T @ _round(B, prec)
+        tol = 4 * U32 * np.sqrt(m)
+    scale = np.abs(Q.astype(np.float64)).T @ np.abs(B.astype(np.float64))
+    err = np.linalg.norm(Cg - ref) / np.linalg.norm(scale)
+    assert err < tol, (name, err)
+
+
+@pytest.mark.parametrize("m,n1,n2", [(4096, 128, 128), (1000, 100, 36), (16384, 32, 32), (300, 7, 5),
+                                     (2048, 640, 256)])
+@pytest.mark.parametrize("prec,name", PRECS)
+def test_gemm_update(later, oracle, m, n1, n2, prec, name):
+    Q = oracle.gen_uniform(m, n1, 1, 0) * np.float32(0.05)
+    C = oracle.gen_uniform(n1, n2, 3, 0)
+    B = oracle.gen_uniform(m, n2, 2, 0)
+    Bd = _dev(later, B)
+    later.gemm_update(_dev(later, Q), _dev(later, C), Bd, prec)
+    _sync()
+    Bg = _np(Bd).astype(np.float64)
+    if prec == 3:
+        ref = B.astype(np.float64) - Q.astype(np.float64) @ C.astype(np.float64)
+        tol = 4e-6
+    else:
+        ref = B.astype(np.float64) - _round(Q, prec) @ _round(C, prec)
+        tol = 4 * U32 * np.sqrt(n1)
+    scale = np.abs(B.astype(np.float64)) + np.abs(Q.astype(np.float64)) @ np.abs(C.astype(np.float64))
+    err = np.linalg.norm(Bg - ref) / np.linalg.norm(scale)
+    assert err < tol, (name, err)
+
+
+# ---------------------------------------------------------------- panel
+@pytest.mark.parametrize("m,w", [(256, 32), (4096, 32), (16384, 32), (1000, 32), (287, 32), (300, 7),
+                                 (64, 1), (70000, 32)])
+@pytest.mark.parametrize("fp64", [False, True])
+def test_caqr_panel(later, oracle, m, w, fp64):
+    A = oracle.gen_uniform(m, w, 0, 0)
+    Ad = _dev(later, A)
+    R = later.caqr_panel(Ad, fp64=fp64)
+    _sync()
+    Qg, Rg = _np(Ad), _np(R)
+    Qo, Ro = oracle.caqr32(A)
+    Q64, R64 = oracle.lapack_qr64(A)
+    assert oracle.rel_err(Rg, Ro) < 2e-6
+    assert oracle.rel_err(Qg, Qo) < 4e-6
+    assert oracle.rel_err(Rg, R64) < 2e-6
+    assert np.array_equal(Rg, np.triu(Rg)) and np.all(np.diag(Rg) > 0)
+    assert oracle.orth(Qg) < (2e-6 if fp64 else 4e-6) * np.sqrt(w)
+
+
+# ---------------------------------------------------------------- rgsqrf
+def _run(later, A, form, **over):
+    Ad = _dev(later, A)
+    o = later.opts(form, **over)
+    R, info = later.rgsqrf(Ad, o=o, info=True)
+    _sync()
+    assert info.status == 0
+    return _np(Ad), _np(R), info
+
+
+@pytest.mark.parametrize("m,n", [(1024, 128), (2048, 256), (4096, 512), (1000, 100), (300, 33), (256, 256)])
+def test_rgsqrf_fp32_form(later, oracle, m, n):
+    A = oracle.gen_uniform(m, n, 0, 0)
+    Q, R, _ = _run(later, A, "fp32")
+    Qo, Ro = oracle.rgsqrf(A, "f32")
+    Q64, R64 = oracle.lapack_qr64(A)
+    kappa = np.linalg.cond(A.astype(np.float64))
+    assert oracle.rel_err(R, Ro) < 2e-6 * max(1.0, kappa / 100)
+    assert oracle.rel_err(R, R64) < 2e-6 * max(1.0, kappa / 100)
+    assert oracle.rel_err(Q, Q64) < 2e-6 * max(1.0, kappa / 10)
+    assert oracle.backward(A, Q, R) < 1e-6
+    assert np.array_equal(R, np.triu(R))
+
+
+@pytest.mark.parametrize("m,n", [(2048, 256), (4096, 512), (1000, 200)])
+def test_rgsqrf_reference_form_f16(later, oracle, m, n):
+    """FP16 node GEMMs (QR/later_rgsqrf.cu:43-56): GPU vs the restatement."""
+    A = oracle.gen_uniform(m, n, 0, 0)
+    Q, R, _ = _run(later, A, "reference")
+    Qo, Ro = oracle.rgsqrf(A, "f16")
+    Q64, R64 = oracle.lapack_qr64(A)
+    e_gpu, e_or = oracle.rel_err(R, R64), oracle.rel_err(Ro, R64)
+    # Both carry the FP16 operand error (~u16 = 4.9e-4 scaled); they agree
+    # with each other far below it.
+    assert oracle.rel_err(R, Ro) < 0.2 * e_or + 1e-5
+    assert e_gpu < 2 * e_or + 1e-5
+    assert abs(oracle.backward(A, Q, R) - oracle.backward(A, Qo, Ro)) < 0.5 * oracle.backward(A, Qo, Ro)
+
+
+def test_rgsqrf_c1_reference_config(later, oracle):
+    """Config C1: m=8192 n=1024 U[-1,1] seed 0, cutoff 128, FP16 GEMMs."""
+    m, n = 8192, 1024
+    Ad = later.generate_uniform(m, n, 0, 0)
+    A = _np(Ad)
+    R = later.rgsqrf(Ad, o=later.opts("reference"))
+    Q, Rg = _np(Ad), _np(R)
+    Qo, Ro = oracle.rgsqrf(A, "f16")
+    assert oracle.rel_err(Rg, Ro) < 3e-5
+    bg, bo = oracle.backward(A, Q, Rg), oracle.backward(A, Qo, Ro)
+    assert bg < 1.5 * bo
+    assert oracle.orth(Q) < 1.5 * oracle.orth(Qo)
+
+
+@pytest.mark.parametrize("m,n,kind", [(2048, 2048, "normal"), (4096, 1024, "uniform"), (1000, 300, "uniform")])
+def test_rgsqrf_mixed_form(later, oracle, m, n, kind):
+    A = oracle.gen_normal(m, n, 0, 0) if kind == "normal" else oracle.gen_uniform(m, n, 0, 0)
+    Q, R, _ = _run(later, A, "mixed")
+    Q64, R64 = oracle.lapack_qr64(A)
+    Qh = np.linalg.qr(A)[0]
+    orth_h = oracle.orth(Qh)  # FP32 Householder's own orthogonality
+    assert oracle.orth(Q) < 3 * orth_h + 1e-6
+    assert oracle.backward(A, Q, R) < 1e-6
+    kappa = np.linalg.cond(A.astype(np.float64))
+    assert oracle.rel_err(R, R64) < 2e-6 * max(1.0, kappa / 100)
+
+
+def test_rgsqrf_graded_mixed_meets_gates(later, oracle):
+    """A = U diag(s) V^T, s geometric 1..1e-4 (config C2 (ii) at n=1024)."""
+    n = 1024
+    rng = np.random.default_rng(0)
+    U, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = 10.0 ** (-4 * np.arange(n) / (n - 1))
+    A = np.asfortranarray((U * s) @ V.T, dtype=np.float32)
+    Q, R, _ = _run(later, A, "mixed")
+    assert oracle.orth(Q) < 1e-5
+    assert oracle.backward(A, Q, R) < 1e-6
+    Qf, Rf, _ = _run(later, A, "fp32")
+    assert oracle.orth(Qf) > 1e-4  # one pass of block CGS loses orthogonality here
+
+
+def test_rgsqrf_deterministic(later, oracle):
+    A = oracle.gen_uniform(4096, 512, 0, 0)
+    for form in ("reference", "fp32", "mixed"):
+        Q1, R1, _ = _run(later, A, form)
+        Q2, R2, _ = _run(later, A, form)
+        assert np.array_equal(Q1, Q2) and np.array_equal(R1, R2), form
+
+
+def test_rgsqrf_honours_lda(later, oracle):
+    import torch
+    m, n, ld = 1000, 200, 1037
+    A = oracle.gen_uniform(m, n, 0, 0)
+    Q1, R1, _ = _run(later, A, "reference")
+    big = later.colmajor(m, n, ld=ld)
+    big.copy_(torch.from_numpy(A))
+    R = later.rgsqrf(big, o=later.opts("reference"))
+    _sync()
+    assert big.stride(1) == ld
+    assert np.array_equal(_np(big), Q1) and np.array_equal(_np(R), R1)
+
+
+def test_legacy_entry_point_matches_ex(later, oracle):
+    import torch
+    m, n = 2048, 256
+    A = oracle.gen_uniform(m, n, 0, 0)
+    Q1, R1, _ = _run(later, A, "reference")
+    Ad = _dev(later, A)
+    R = later.colmajor(n, n)
+    R.fill_(7.0)
+    work = torch.empty(m // 256 * 32 * n, device="cuda")     # test/test_qr.cu:73
+    hwork = torch.empty(m * n, device="cuda", dtype=torch.int16)  # test/test_qr.cu:76
+    later.rgsqrf_legacy(Ad, R, work, hwork)
+    _sync()
+    assert np.array_equal(_np(Ad), Q1) and np.array_equal(_np(R), R1)
+
+
+def test_breakdown_sets_status(later, oracle):
+    import torch
+    A = oracle.gen_uniform(512, 64, 0, 0)
+    A[:, 5] = 0.0
+    Ad = _dev(later, A)
+    o = later.opts("reference")
+    work = later.workspace(512, 64, o)
+    R = later.rgsqrf(Ad, o=o, work=work)
+    assert later.read_status(work) & 1
+    assert torch.isfinite(Ad).all() and torch.isfinite(R).all()
+    ok = later.workspace(512, 64, o)
+    later.rgsqrf(_dev(later, oracle.gen_uniform(512, 64, 0, 0)), o=o, work=ok)
+    assert later.read_status(ok) == 0
+
+
+def test_workspace_too_small_is_an_error(later):
+    import torch
+    A = later.generate_uniform(4096, 512)
+    with pytest.raises(later.LaterError) as e:
+        later.rgsqrf(A, o=later.opts("mixed"), work=torch.empty(10, device="cuda"))
+    assert e.value.code == later.ERR_WORKSPACE
+
+
+# ---------------------------------------------------------------- -check
+@pytest.mark.parametrize("m,n", [(1000, 100), (4096, 256)])
+def test_check_metrics_match_oracle(later, oracle, m, n):
+    A = oracle.gen_uniform(m, n, 0, 0)
+    Q, R = oracle.rgsqrf(A, "f16")
+    Qd, Rd, Ad = _dev(later, Q), _dev(later, R), _dev(later, A)
+    o_g, o_o = later.check_orth(Qd), oracle.orth(Q)
+    b_g, b_o = later.check_backward(Ad, Qd, Rd), oracle.backward(A, Q, R)
+    assert abs(o_g - o_o) < 1e-9 * max(1.0, o_o) + 1e-12
+    assert abs(b_g - b_o) < 1e-6 * b_o + 1e-15
