@@ -71,7 +71,7 @@ enum {
     LATER_PANEL_MGS64 = 1   /* same tree, FP64 column norms and dot products */
 };
 
-/* Re-orthogonalisation. */
+/* Re-orthogonalisation (bit flags; NODE|PASS = 3 combines both). */
 enum {
     LATER_REORTH_NONE = 0,  /* single pass (the reference) */
     LATER_REORTH_NODE = 1,  /* second projection at every node (block CGS2) */
@@ -140,6 +140,44 @@ int later_gemm_proj(laterCtxt ctxt, int m, int n1, int n2, const float *Q, int l
 int later_gemm_update(laterCtxt ctxt, int m, int n1, int n2, const float *Q, int ldq,
                       const float *C, int ldc, float *B, int ldb, int prec, float *work,
                       int64_t lwork);
+
+/* Least squares (the LS-solve of the serving path): triangular solve
+ * B (n x k, ldb) <- R^{-1} B with R upper triangular (n <= 8192), FP64 sums;
+ * stands in for the reference's recursive TRSM (BLAS/later_rtrsm.cu:75-187). */
+int later_trsm_upper(laterCtxt ctxt, int n, int k, const float *R, int ldr, float *B, int ldb);
+
+/* min ||A X - B||_2: A (m x n) <- Q, R (n x n) <- R, X (n x k) <- R^{-1} Q^T B.
+ * B is read only; Q^T B uses FP16 hi/lo products with FP64-folded chunks.
+ * later_ls_workspace gives the FP32 workspace for (m, n, k, opts). */
+int later_ls_workspace(int m, int n, int k, const later_opts *opts, int64_t *lwork);
+int later_gels(laterCtxt ctxt, int m, int n, int k, float *A, int lda, float *R, int ldr,
+               const float *B, int ldb, float *X, int ldx, float *work, int64_t lwork,
+               const later_opts *opts, later_info *info);
+
+/* Least-squares serving runtime (continuous batching on one GPU).
+ * later_server_create allocates the HBM pool of `hbm_budget_bytes` once
+ * (the only allocation; requests never allocate) and `slots` streams.
+ * later_server_submit queues a request (device pointers A: m x n, B: m x k,
+ * owned by the caller until the request is done); it is rejected with
+ * LATER_ERR_BUDGET when its working set exceeds the pool.  Requests are
+ * admitted in arrival order as slots and pool memory free up and run
+ * concurrently, each on its own stream.  later_server_poll advances every
+ * request without blocking.  Results (Q, R, X) stay resident in the pool
+ * until later_server_release.  States: 0 waiting, 1 QR, 2 solve, 3 done,
+ * 4 failed, 5 robust re-factorisation (after an orthogonality sketch above
+ * 1e-5).  Calls on one server are serialised by an internal mutex. */
+int later_server_create(int slots, int64_t hbm_budget_bytes, const later_opts *qr_opts, void **srv);
+int later_server_submit(void *srv, int m, int n, int k, const float *A, int lda, const float *B, int ldb,
+                        uint64_t *id);
+int later_server_poll(void *srv, int *active);
+int later_server_result(void *srv, uint64_t id, int *state, int *status, const float **X,
+                        const float **Q, const float **R, double *latency_s, double *queue_s,
+                        double *cond, int *iters);
+int later_server_fetch(void *srv, uint64_t id, float *X, int ldx, float *R, int ldr, float *Q, int ldq);
+int later_server_release(void *srv, uint64_t id);
+int later_server_stats(void *srv, int64_t *pool_bytes, int64_t *used_bytes, int64_t *peak_bytes,
+                       int *completed, int *rejected);
+int later_server_destroy(void *srv);
 
 /* Input generator (replaces util/util.cu:93-109, which uses cuRAND):
  * Philox4x32-10 keyed by (seed, mat), counter = column-major index i + j*m.

@@ -77,6 +77,21 @@ _SIGNATURES = {
     "later_check_orth": (_i, [laterCtxt, _i, _i, _vp, _i, ctypes.POINTER(_d)]),
     "later_check_backward": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _vp, _i, ctypes.POINTER(_d)]),
     "later_version": (ctypes.c_char_p, []),
+    "later_trsm_upper": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i]),
+    "later_ls_workspace": (_i, [_i, _i, _i, ctypes.POINTER(later_opts), ctypes.POINTER(_i64)]),
+    "later_gels": (_i, [laterCtxt, _i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i64,
+                        ctypes.POINTER(later_opts), ctypes.POINTER(later_info)]),
+    "later_server_create": (_i, [_i, _i64, ctypes.POINTER(later_opts), ctypes.POINTER(_vp)]),
+    "later_server_submit": (_i, [_vp, _i, _i, _i, _vp, _i, _vp, _i, ctypes.POINTER(ctypes.c_uint64)]),
+    "later_server_poll": (_i, [_vp, ctypes.POINTER(_i)]),
+    "later_server_result": (_i, [_vp, ctypes.c_uint64, ctypes.POINTER(_i), ctypes.POINTER(_i),
+                                 ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                 ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_i)]),
+    "later_server_release": (_i, [_vp, ctypes.c_uint64]),
+    "later_server_fetch": (_i, [_vp, ctypes.c_uint64, _vp, _i, _vp, _i, _vp, _i]),
+    "later_server_stats": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "later_server_destroy": (_i, [_vp]),
     "later_prof_begin": (None, []),
     "later_prof_end": (_i, [ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
 }
@@ -88,6 +103,9 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run `python later_amd/build.py`")
+        # torch ships its own libamdhip64.so.7; loading it first makes the
+        # library bind to that same HIP runtime (one runtime per process).
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGNATURES.items():
             fn = getattr(L, name)
@@ -262,6 +280,95 @@ def check_backward(A, Q, R, stream=None) -> float:
     _check(lib().later_check_backward(ctxt(stream), m, n, A.data_ptr(), _ld(A), Q.data_ptr(), _ld(Q),
                                       R.data_ptr(), _ld(R), ctypes.byref(out)), "later_check_backward")
     return float(out.value)
+
+
+# --------------------------------------------------------------------------
+# Least squares and serving
+# --------------------------------------------------------------------------
+def trsm_upper(R, B, stream=None):
+    """B[:n] <- R^{-1} B[:n] for upper-triangular R (n x n)."""
+    n = R.shape[0]
+    k = B.shape[1]
+    _check(lib().later_trsm_upper(ctxt(stream), n, k, R.data_ptr(), _ld(R), B.data_ptr(), _ld(B)),
+           "later_trsm_upper")
+    return B
+
+
+def gels(A, B, o: later_opts | None = None, stream=None):
+    """min ||A X - B||: A <- Q in place; returns (X, R)."""
+    torch = _torch()
+    m, n = A.shape
+    k = B.shape[1]
+    lw = _i64()
+    _check(lib().later_ls_workspace(m, n, k, ctypes.byref(o) if o is not None else None, ctypes.byref(lw)),
+           "later_ls_workspace")
+    work = torch.empty(max(lw.value, 1), device=A.device, dtype=torch.float32)
+    R = colmajor(n, n, device=A.device)
+    X = colmajor(n, k, device=A.device)
+    _check(lib().later_gels(ctxt(stream), m, n, k, A.data_ptr(), _ld(A), R.data_ptr(), _ld(R), B.data_ptr(),
+                            _ld(B), X.data_ptr(), _ld(X), work.data_ptr(), work.numel(),
+                            ctypes.byref(o) if o is not None else None, None), "later_gels")
+    return X, R
+
+
+class Server:
+    """Continuous-batching least-squares server (later_server_*)."""
+
+    def __init__(self, slots: int = 8, hbm_budget_bytes: int = 8 << 30, o: later_opts | None = None):
+        h = _vp()
+        _check(lib().later_server_create(slots, hbm_budget_bytes, ctypes.byref(o) if o is not None else None,
+                                         ctypes.byref(h)), "later_server_create")
+        self.h = h
+
+    def submit(self, A, B) -> int:
+        m, n = A.shape
+        k = B.shape[1]
+        rid = ctypes.c_uint64()
+        rc = lib().later_server_submit(self.h, m, n, k, A.data_ptr(), _ld(A), B.data_ptr(), _ld(B),
+                                       ctypes.byref(rid))
+        if rc not in (OK, ERR_BUDGET):
+            _check(rc, "later_server_submit")
+        return int(rid.value)
+
+    def poll(self) -> int:
+        a = _i()
+        _check(lib().later_server_poll(self.h, ctypes.byref(a)), "later_server_poll")
+        return int(a.value)
+
+    def result(self, rid: int) -> dict:
+        st, status, it = _i(), _i(), _i()
+        X, Q, R = _vp(), _vp(), _vp()
+        lat, qs, cond = _d(), _d(), _d()
+        _check(lib().later_server_result(self.h, rid, ctypes.byref(st), ctypes.byref(status), ctypes.byref(X),
+                                         ctypes.byref(Q), ctypes.byref(R), ctypes.byref(lat), ctypes.byref(qs),
+                                         ctypes.byref(cond), ctypes.byref(it)), "later_server_result")
+        return {"state": st.value, "status": status.value, "X": X.value, "Q": Q.value, "R": R.value,
+                "latency_s": lat.value, "queue_s": qs.value, "cond": cond.value, "iters": it.value}
+
+    def fetch(self, rid: int, m: int, n: int, k: int, q: bool = False):
+        """Copies of (X, R[, Q]) of a finished request into new tensors."""
+        X = colmajor(n, k)
+        R = colmajor(n, n)
+        Q = colmajor(m, n) if q else None
+        _check(lib().later_server_fetch(self.h, rid, X.data_ptr(), _ld(X), R.data_ptr(), _ld(R),
+                                        Q.data_ptr() if q else None, _ld(Q) if q else 0), "later_server_fetch")
+        return (X, R, Q) if q else (X, R)
+
+    def release(self, rid: int):
+        _check(lib().later_server_release(self.h, rid), "later_server_release")
+
+    def stats(self) -> dict:
+        pb, ub, pk = _i64(), _i64(), _i64()
+        c, r = _i(), _i()
+        _check(lib().later_server_stats(self.h, ctypes.byref(pb), ctypes.byref(ub), ctypes.byref(pk),
+                                        ctypes.byref(c), ctypes.byref(r)), "later_server_stats")
+        return {"pool_bytes": pb.value, "used_bytes": ub.value, "peak_bytes": pk.value, "completed": c.value,
+                "rejected": r.value}
+
+    def close(self):
+        if self.h:
+            _check(lib().later_server_destroy(self.h), "later_server_destroy")
+            self.h = None
 
 
 PROF_CLASSES = ("gemm_proj", "gemm_update", "panel_mgs", "other")

@@ -115,27 +115,64 @@ template <int PREC, int R> struct Tile {
     static constexpr bool F32 = (PREC == GEMM_F32);
     static constexpr int LD = F32 ? (R + 4) : (BK + 8);
     static constexpr int SZ = F32 ? BK * LD : R * LD;
-    T hi[SZ];
-    T lo[PREC == GEMM_F16X3 ? SZ : 1];
+    alignas(16) T hi[SZ];
+    alignas(16) T lo[PREC == GEMM_F16X3 ? SZ : 8];
 
     __device__ __forceinline__ int idx(int row, int k) const { return F32 ? k * LD + row : row * LD + k; }
 };
 
+template <typename T, int N> struct Vec {
+    typedef T type __attribute__((ext_vector_type(N)));
+};
+
+// Writes a thread's slab to LDS, converted to the MFMA input type, with the
+// widest stores the layout allows: 16-bit types keep k contiguous per row, so
+// a K-major slab (SEG consecutive k of one row) goes out as 16-byte vectors
+// and an M-major slab (KG consecutive k for each of 4 rows) as KG-wide ones;
+// FP32 keeps rows contiguous per k, so an M-major slab writes 4-row vectors.
 template <int PREC, int R, bool KM>
 __device__ __forceinline__ void store_slab(Tile<PREC, R> &t, const Slab<R, KM> &s, float scale)
 {
     typedef typename Lds<PREC>::T T;
+    typedef Slab<R, KM> S;
+    if constexpr (PREC == GEMM_F32) {
+        if constexpr (KM) {
 #pragma unroll
-    for (int e = 0; e < Slab<R, KM>::NV; ++e) {
-        const int row = s.row_of(e), k = s.k_of(e);
-        float x = s.v[e] * scale;
-        if constexpr (PREC == GEMM_F16X3) {
-            _Float16 h = (_Float16)x;
-            _Float16 l = (_Float16)((x - (float)h) * 2048.0f);
-            t.hi[t.idx(row, k)] = h;
-            t.lo[t.idx(row, k)] = l;
+            for (int e = 0; e < S::NV; ++e) t.hi[t.idx(s.row_of(e), s.k_of(e))] = s.v[e] * scale;
         } else {
-            t.hi[t.idx(row, k)] = (T)x;
+#pragma unroll
+            for (int kk = 0; kk < S::KG; ++kk) {
+                floatx4 x;
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) x[rr] = s.v[kk * 4 + rr] * scale;
+                *(floatx4 *)&t.hi[t.idx(s.row_of(kk * 4), s.k_of(kk * 4))] = x;
+            }
+        }
+    } else {
+        constexpr int W = KM ? 8 : S::KG;      // elements per store
+        constexpr int NST = S::NV / W;          // stores per array
+        typedef typename Vec<T, W>::type VT;
+#pragma unroll
+        for (int q = 0; q < NST; ++q) {
+            VT hv, lv;
+            int row, k;
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                // KM: values q*W .. q*W+W-1 are consecutive k of one row.
+                // M-major: value kk*4+rr is row rr, k kk; store q = row rr.
+                const int idx = KM ? q * W + e : e * 4 + q;
+                float x = s.v[idx] * scale;
+                if constexpr (PREC == GEMM_F16X3) {
+                    _Float16 h = (_Float16)x;
+                    hv[e] = h;
+                    lv[e] = (_Float16)((x - (float)h) * 2048.0f);
+                } else {
+                    hv[e] = (T)x;
+                }
+                if (e == 0) { row = s.row_of(idx); k = s.k_of(idx); }
+            }
+            *(VT *)&t.hi[t.idx(row, k)] = hv;
+            if constexpr (PREC == GEMM_F16X3) *(VT *)&t.lo[t.idx(row, k)] = lv;
         }
     }
 }
@@ -177,8 +214,9 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
     constexpr bool X3 = (PREC == GEMM_F16X3);
     constexpr int KSTEPS = (PREC == GEMM_F32) ? BK / 2 : BK / 16;
 
-    __shared__ Tile<PREC, BM> As;
-    __shared__ Tile<PREC, BN> Bs;
+    // Double-buffered operand tiles: one barrier per BK slab.
+    __shared__ Tile<PREC, BM> As[2];
+    __shared__ Tile<PREC, BN> Bs[2];
 
     const int tm = blockIdx.x, tn = blockIdx.y, split = blockIdx.z;
     const int m0 = tm * BM, n0 = tn * BN;
@@ -205,6 +243,9 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
     const int bcol = n0 + sb.row_of(0);
     const float bscale = (g.b_exp && bcol < g.N) ? ldexpf(1.0f, g.b_exp[bcol]) : 1.0f;
 
+    // The MFMA computes the transposed tile (B operand rows x A operand rows),
+    // so that lane l%32 walks the rows of C: stores of column-major C are
+    // then 128-byte contiguous per half-wave.
     floatx16 acc[FM][FN], accl[X3 ? FM : 1][X3 ? FN : 1];
     double acc64[CH64 ? FM : 1][CH64 ? FN : 1][CH64 ? 16 : 1];
 #pragma unroll
@@ -243,12 +284,14 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
     if (ntiles > 0) {
         sa.load(g.A, g.lda, m0, g.M, kbeg, kend, avec);
         sb.load(g.B, g.ldb, n0, g.N, kbeg, kend, bvec);
+        store_slab<PREC, BM, AKM>(As[0], sa, ascale);
+        store_slab<PREC, BN, true>(Bs[0], sb, bscale);
+        __syncthreads();
     }
     for (int it = 0; it < ntiles; ++it) {
-        store_slab<PREC, BM, AKM>(As, sa, ascale);
-        store_slab<PREC, BN, true>(Bs, sb, bscale);
-        __syncthreads();
-        if (it + 1 < ntiles) {
+        const int cur = it & 1;
+        const bool more = it + 1 < ntiles;
+        if (more) {
             const int k0 = kbeg + (it + 1) * BK;
             sa.load(g.A, g.lda, m0, g.M, k0, kend, avec);
             sb.load(g.B, g.ldb, n0, g.N, k0, kend, bvec);
@@ -259,43 +302,53 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
             typename Frag<PREC>::T fal[X3 ? FM : 1], fbl[X3 ? FN : 1];
 #pragma unroll
             for (int i = 0; i < FM; ++i) {
-                fa[i] = frag<PREC, BM>(As.hi, wm * WM + 32 * i, ks);
-                if constexpr (X3) fal[i] = frag<PREC, BM>(As.lo, wm * WM + 32 * i, ks);
+                fa[i] = frag<PREC, BM>(As[cur].hi, wm * WM + 32 * i, ks);
+                if constexpr (X3) fal[i] = frag<PREC, BM>(As[cur].lo, wm * WM + 32 * i, ks);
             }
 #pragma unroll
             for (int j = 0; j < FN; ++j) {
-                fb[j] = frag<PREC, BN>(Bs.hi, wn * WN + 32 * j, ks);
-                if constexpr (X3) fbl[j] = frag<PREC, BN>(Bs.lo, wn * WN + 32 * j, ks);
+                fb[j] = frag<PREC, BN>(Bs[cur].hi, wn * WN + 32 * j, ks);
+                if constexpr (X3) fbl[j] = frag<PREC, BN>(Bs[cur].lo, wn * WN + 32 * j, ks);
             }
+            // Grouped by product so that updates of one accumulator are
+            // separated by independent MFMAs.
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int j = 0; j < FN; ++j) {
-                    acc[i][j] = mfma<PREC>(fa[i], fb[j], acc[i][j]);
-                    if constexpr (X3) {
-                        accl[i][j] = mfma<PREC>(fa[i], fbl[j], accl[i][j]);
-                        accl[i][j] = mfma<PREC>(fal[i], fb[j], accl[i][j]);
-                    }
-                }
+                for (int j = 0; j < FN; ++j) acc[i][j] = mfma<PREC>(fb[j], fa[i], acc[i][j]);
+            if constexpr (X3) {
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) accl[i][j] = mfma<PREC>(fbl[j], fa[i], accl[i][j]);
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) accl[i][j] = mfma<PREC>(fb[j], fal[i], accl[i][j]);
+            }
         }
         if (CH64 && ((it + 1) % g.chunk_tiles == 0 || it + 1 == ntiles)) fold();
+        if (more) {
+            store_slab<PREC, BM, AKM>(As[cur ^ 1], sa, ascale);
+            store_slab<PREC, BN, true>(Bs[cur ^ 1], sb, bscale);
+        }
         __syncthreads();
     }
 
-    // Epilogue.  32x32 accumulator layout: lane l holds column l%32 and rows
-    // 8 (r/4) + 4 (l/32) + r%4, r = 0..15.
+    // Epilogue.  Transposed 32x32 accumulator layout: lane l holds row l%32
+    // of the block and columns 8 (r/4) + 4 (l/32) + r%4, r = 0..15.
     const float oscale_a = ldexpf(1.0f, -g.a_exp);
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i) {
+        const int row = m0 + wm * WM + 32 * i + (lane & 31);
+        if (row >= g.M) continue;
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-            const int col = n0 + wn * WN + 32 * j + (lane & 31);
-            if (col >= g.N) continue;
-            const float cs = oscale_a * ((g.b_exp) ? ldexpf(1.0f, -g.b_exp[col]) : 1.0f);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int row = m0 + wm * WM + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (row >= g.M) continue;
+                const int col = n0 + wn * WN + 32 * j + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (col >= g.N) continue;
+                const float cs = oscale_a * ((g.b_exp) ? ldexpf(1.0f, -g.b_exp[col]) : 1.0f);
                 double v;
                 if constexpr (CH64) v = acc64[i][j][r];
                 else if constexpr (X3) v = (double)acc[i][j][r] + (double)accl[i][j][r] * 0x1p-11;
@@ -314,6 +367,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
                 }
             }
         }
+    }
 }
 
 // C = sum over splits (fixed order, FP64) of the partials.

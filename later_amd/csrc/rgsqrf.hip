@@ -55,11 +55,11 @@ Layout layout(int m, int n, const later_opts &o)
     L.panel = panel_work_floats(m, 32);
     int64_t g = 0;
     walk_gemm(m, n, o.nmin, o.chunk, &g);
-    if (o.reorth == LATER_REORTH_PASS) walk_gemm(m, n, o.nmin, o.chunk, &g);
+    if (o.reorth & LATER_REORTH_PASS) walk_gemm(m, n, o.nmin, o.chunk, &g);
     L.gemm = g;
     const int n1 = split_n(n);
-    L.snode = (o.reorth == LATER_REORTH_NODE && n > 32) ? (int64_t)n1 * (n - n1) : 0;
-    L.r2 = (o.reorth == LATER_REORTH_PASS) ? (int64_t)n * n : 0;
+    L.snode = ((o.reorth & LATER_REORTH_NODE) && n > 32) ? (int64_t)n1 * (n - n1) : 0;
+    L.r2 = (o.reorth & LATER_REORTH_PASS) ? (int64_t)n * n : 0;
     L.tcopy = L.r2;
     L.total = L.header + L.exps + L.panel + L.gemm + L.snode + L.r2 + L.tcopy;
     return L;
@@ -75,6 +75,7 @@ struct Run {
     float *snode;
     int launches;
     int depth;
+    bool node_reorth;  // block CGS2 at the nodes of this pass
 };
 
 __global__ void zero_lower_kernel(int n, float *R, int64_t ldr)
@@ -146,7 +147,7 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     }
     rc = gemm_update(r.s, p, m, n1, n2, A, lda, R12, ldr, A2, lda, f64, bexp, aexp, &r.launches);
     if (rc) return rc;
-    if (r.o.reorth == LATER_REORTH_NODE) {
+    if (r.node_reorth) {
         // S = Q1^T A2; A2 -= Q1 S; R12 += S
         float *S = r.snode;
         if (scaled(p)) {
@@ -225,7 +226,7 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     int rc = LATER_OK;
     if (m < 1 || n < 1 || m < n || lda < m || ldr < n || !A || !R) rc = LATER_ERR_SHAPE;
     else if (!prec_ok(o.gemm) || !prec_ok(o.gemm_pass2) || o.panel < 0 || o.panel > 1 || o.reorth < 0 ||
-             o.reorth > 2 || o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0)
+             o.reorth > 3 || o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0)
         rc = LATER_ERR_OPTION;
     Layout L = layout(m > 0 ? m : 1, n > 0 ? n : 1, o);
     // Everything lives in `work` when it is large enough; otherwise the
@@ -266,16 +267,18 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     float *T = p; p += L.tcopy;
     r.launches = 0;
     r.depth = 0;
+    r.node_reorth = (o.reorth & LATER_REORTH_NODE) != 0;
 
     hipLaunchKernelGGL(status_clear_kernel, dim3(1), dim3(64), 0, r.s, r.status);
     hipLaunchKernelGGL(zero_lower_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R, (int64_t)ldr);
     r.launches += 2;
     rc = hipGetLastError() == hipSuccess ? LATER_OK : LATER_ERR_DEVICE;
     if (!rc) rc = qr_rec(r, m, n, A, lda, R, ldr, o.gemm, 0);
-    if (!rc && o.reorth == LATER_REORTH_PASS) {
+    if (!rc && (o.reorth & LATER_REORTH_PASS)) {
         // Second pass on Q, then R = R1 + (R2 - I) R1 (upper-triangular product).
         hipLaunchKernelGGL(fill_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, (int64_t)n * n, R2, 0.0f);
         ++r.launches;
+        r.node_reorth = false;  // Q has condition number ~1: one projection per node suffices
         rc = qr_rec(r, m, n, A, lda, R2, n, o.gemm_pass2, 0);
         if (!rc) {
             hipLaunchKernelGGL(copy_upper_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R, (int64_t)ldr, T,
