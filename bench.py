@@ -39,26 +39,41 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_form(later, torch, A0, form, steps, warmup):
-    """Times `steps` factorisations of a fresh copy of A0; returns ms list and the last (Q, R)."""
+def time_form(later, torch, A0, form, steps, warmup, graph=True):
+    """Times `steps` factorisations, each of a fresh copy of A0 (the copy is
+    outside the timed events).  With graph=True the factorisation is one
+    replay of a graph-captured plan; the first eager call is timed too."""
     m, n = A0.shape
     o = later.opts(form)
     work = later.workspace(m, n, o)
     A = later.colmajor(m, n)
     R = later.colmajor(n, n)
     s = torch.cuda.current_stream()
-    times = []
-    for it in range(warmup + steps):
+
+    def one(run):
         A.copy_(A0)
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         ev0.record(s)
-        later.rgsqrf(A, R=R, o=o, work=work)
+        run()
         ev1.record(s)
         torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1)
+
+    eager_ms = one(lambda: later.rgsqrf(A, R=R, o=o, work=work))
+    run = lambda: later.rgsqrf(A, R=R, o=o, work=work)
+    plan = None
+    if graph:
+        plan = later.Plan(A, R, o, work)
+        run = plan.execute
+    times = []
+    for it in range(warmup + steps):
+        t = one(run)
         if it >= warmup:
-            times.append(ev0.elapsed_time(ev1))
-    return times, A, R, work, o
+            times.append(t)
+    if plan is not None:
+        plan.close()
+    return times, eager_ms, A, R, work, o
 
 
 def profile_step(later, torch, A0, form, work, o):
@@ -97,6 +112,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this QR path runs on one GPU; multi-GPU row sharding is described "
@@ -115,9 +131,10 @@ def main():
     res = {}
     for form in ("fp32", "mixed"):
         t0 = time.time()
-        times, Q, R, work, o = time_form(later, torch, A0, form, args.steps, args.warmup)
+        times, eager_ms, Q, R, work, o = time_form(later, torch, A0, form, args.steps, args.warmup,
+                                                   graph=not args.no_graph)
         ms = sum(times) / len(times)
-        entry = {"ms": ms, "ms_all": times, "tflops_ref": later.flops_ref(m, n) / ms / 1e9,
+        entry = {"ms": ms, "ms_all": times, "eager_ms": eager_ms, "tflops_ref": later.flops_ref(m, n) / ms / 1e9,
                  "tflops_mgs": later.flops_mgs(m, n) / ms / 1e9}
         if not args.no_check:
             entry["orth"] = later.check_orth(Q)
@@ -125,7 +142,7 @@ def main():
             entry["backward"] = later.check_backward(A0, Q, R)
         entry["prof"] = profile_step(later, torch, A0, form, work, o)
         res[form] = entry
-        log(f"[bench] {form}: {ms:.2f} ms  {entry['tflops_ref']:.1f} TFLOP/s(ref)  "
+        log(f"[bench] {form}: {ms:.2f} ms (eager {eager_ms:.2f})  {entry['tflops_ref']:.1f} TFLOP/s(ref)  "
             f"orth={entry.get('orth', float('nan')):.3e} bwd={entry.get('backward', float('nan')):.3e} "
             f"({time.time() - t0:.1f} s)")
 
@@ -161,9 +178,12 @@ def main():
         "data": "synthetic: Philox N(0,1) seed 0, generated on device",
         "config": {"workload": f"square QR m={m} n={n} FP32 column-major N(0,1), mixed-precision rgsqrf "
                                f"with second Gram-Schmidt pass", "m": m, "n": n},
-        "fp32_form": {"tflops_ref": fp32["tflops_ref"], "ms": fp32["ms"], "orth": fp32.get("orth"),
+        "timing": "graph replay of the captured factorisation" if not args.no_graph else "eager launches",
+        "fp32_form": {"tflops_ref": fp32["tflops_ref"], "ms": fp32["ms"], "eager_ms": fp32["eager_ms"],
+                      "orth": fp32.get("orth"),
                       "backward": fp32.get("backward")},
         "mixed_form": {"tflops_ref": mixed["tflops_ref"], "tflops_mgs": mixed["tflops_mgs"],
+                       "eager_ms": mixed["eager_ms"],
                        "orth": mixed.get("orth"), "orth_over_n": mixed.get("orth_over_n"),
                        "backward": mixed.get("backward")},
         "speedup_vs_fp32": fp32["ms"] / mixed["ms"],

@@ -61,8 +61,10 @@ enum {
     LATER_GEMM_F16 = 1,   /* FP16 operands, FP32 accumulate: the reference's
                              GemmEx TENSOR_OP pair (QR/later_rgsqrf.cu:45-56) */
     LATER_GEMM_BF16 = 2,  /* BF16 operands, FP32 accumulate */
-    LATER_GEMM_F16X3 = 3  /* FP16 hi/lo split of both operands, three MFMA
+    LATER_GEMM_F16X3 = 3, /* FP16 hi/lo split of both operands, three MFMA
                              products, power-of-two column scaling */
+    LATER_GEMM_F16X6 = 4  /* FP16 three-way split (2^-33 representation), six
+                             MFMA products; projections only (updates use X3) */
 };
 
 /* Panel (leaf) algorithm. */
@@ -85,6 +87,7 @@ typedef struct later_opts {
     int reorth;       /* LATER_REORTH_* */
     int nmin;         /* nodes of width <= nmin use FP32 products (reference: 128) */
     int chunk;        /* rows per FP32 accumulation chunk folded into FP64 (0 = off) */
+    int chunk2;       /* the same for the second pass (0 = use chunk) */
 } later_opts;
 
 /* Defaults reproduce the reference form: FP16 GEMMs above 128 columns, FP32
@@ -118,6 +121,18 @@ typedef struct later_info {
 int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, int ldr,
                     float *work, int64_t lwork, uint16_t *hwork, int64_t lhwork,
                     const later_opts *opts, later_info *info);
+
+/* Graph-captured plans (no counterpart in the reference, which launches
+ * eagerly): later_plan_create records every kernel of one later_rgsqrf_ex
+ * call on these exact buffers and options into a HIP graph (it enqueues
+ * nothing and computes nothing); later_plan_execute replays it on `stream`
+ * with one graph launch and the same effect as the eager call.  The buffers
+ * must stay allocated while the plan exists. */
+int later_plan_create(int m, int n, float *A, int lda, float *R, int ldr, float *work, int64_t lwork,
+                      const later_opts *opts, void **plan);
+int later_plan_execute(void *plan, void *stream);
+int later_plan_kernels(void *plan);
+int later_plan_destroy(void *plan);
 
 /* Synchronises ctxt.stream and reads the device status word that the last
  * later_rgsqrf_ex / later_caqr_panel call on this `work` left: bit 0 set

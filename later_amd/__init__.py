@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "liblater_amd.so")
 
 # Status codes (include/later.h)
 OK, ERR_SHAPE, ERR_WORKSPACE, ERR_OPTION, ERR_DEVICE, ERR_BREAKDOWN, ERR_ALIAS, ERR_BUDGET = range(8)
-GEMM_F32, GEMM_F16, GEMM_BF16, GEMM_F16X3 = range(4)
+GEMM_F32, GEMM_F16, GEMM_BF16, GEMM_F16X3, GEMM_F16X6 = range(5)
 PANEL_MGS, PANEL_MGS64 = range(2)
 REORTH_NONE, REORTH_NODE, REORTH_PASS = range(3)
 
@@ -44,11 +44,12 @@ class laterCtxt(ctypes.Structure):
 
 class later_opts(ctypes.Structure):
     _fields_ = [("gemm", ctypes.c_int), ("gemm_pass2", ctypes.c_int), ("panel", ctypes.c_int),
-                ("reorth", ctypes.c_int), ("nmin", ctypes.c_int), ("chunk", ctypes.c_int)]
+                ("reorth", ctypes.c_int), ("nmin", ctypes.c_int), ("chunk", ctypes.c_int),
+                ("chunk2", ctypes.c_int)]
 
     def __repr__(self):
-        return ("later_opts(gemm=%d, gemm_pass2=%d, panel=%d, reorth=%d, nmin=%d, chunk=%d)"
-                % (self.gemm, self.gemm_pass2, self.panel, self.reorth, self.nmin, self.chunk))
+        return ("later_opts(gemm=%d, gemm_pass2=%d, panel=%d, reorth=%d, nmin=%d, chunk=%d, chunk2=%d)"
+                % (self.gemm, self.gemm_pass2, self.panel, self.reorth, self.nmin, self.chunk, self.chunk2))
 
 
 class later_info(ctypes.Structure):
@@ -92,6 +93,10 @@ _SIGNATURES = {
     "later_server_stats": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                 ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "later_server_destroy": (_i, [_vp]),
+    "later_plan_create": (_i, [_i, _i, _vp, _i, _vp, _i, _vp, _i64, ctypes.POINTER(later_opts), ctypes.POINTER(_vp)]),
+    "later_plan_execute": (_i, [_vp, _vp]),
+    "later_plan_kernels": (_i, [_vp]),
+    "later_plan_destroy": (_i, [_vp]),
     "later_prof_begin": (None, []),
     "later_prof_end": (_i, [ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
 }
@@ -193,6 +198,41 @@ def rgsqrf(A, R=None, o: later_opts | None = None, work=None, stream=None, info:
                                ctypes.byref(o) if o is not None else None, ctypes.byref(inf))
     _check(rc, "later_rgsqrf_ex")
     return (R, inf) if info else R
+
+
+class Plan:
+    """A graph-captured factorisation of A (in place) into R for fixed buffers."""
+
+    def __init__(self, A, R, o: later_opts | None = None, work=None):
+        m, n = A.shape
+        self.A, self.R = A, R
+        self.work = work if work is not None else workspace(m, n, o, device=A.device)
+        self.o = o
+        h = _vp()
+        _check(lib().later_plan_create(m, n, A.data_ptr(), _ld(A), R.data_ptr(), _ld(R), self.work.data_ptr(),
+                                       self.work.numel(), ctypes.byref(o) if o is not None else None,
+                                       ctypes.byref(h)), "later_plan_create")
+        self.h = h
+
+    @property
+    def kernels(self) -> int:
+        return int(lib().later_plan_kernels(self.h))
+
+    def execute(self, stream=None):
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _check(lib().later_plan_execute(self.h, ctypes.c_void_p(s.cuda_stream)), "later_plan_execute")
+
+    def close(self):
+        if self.h:
+            lib().later_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def rgsqrf_legacy(A, R, work, hwork, stream=None):

@@ -3,7 +3,7 @@
 #include "common.h"
 
 enum { GEMM_F32 = LATER_GEMM_F32, GEMM_F16 = LATER_GEMM_F16, GEMM_BF16 = LATER_GEMM_BF16,
-       GEMM_F16X3 = LATER_GEMM_F16X3 };
+       GEMM_F16X3 = LATER_GEMM_F16X3, GEMM_F16X6 = LATER_GEMM_F16X6 };
 
 // Floats of workspace the projection needs for its split-K partials.
 int64_t gemm_proj_work_floats(int m, int n1, int n2, int chunk);

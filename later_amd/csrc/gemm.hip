@@ -27,6 +27,13 @@ template <> struct Lds<GEMM_F32> { typedef float T; };
 template <> struct Lds<GEMM_F16> { typedef _Float16 T; };
 template <> struct Lds<GEMM_BF16> { typedef __bf16 T; };
 template <> struct Lds<GEMM_F16X3> { typedef _Float16 T; };
+template <> struct Lds<GEMM_F16X6> { typedef _Float16 T; };
+
+// FP16 pieces per operand value: x = p0 + p1 2^-11 + p2 2^-22, with
+// p0 = fl16(x), p1 = fl16((x - p0) 2^11), p2 = fl16((x - p0 - p1 2^-11) 2^22).
+// Products are summed per power of 2^-11 in separate FP32 accumulators:
+// group g holds sum_{i+j=g} p_i q_j.
+template <int PREC> constexpr int pieces() { return PREC == GEMM_F16X6 ? 3 : PREC == GEMM_F16X3 ? 2 : 1; }
 
 struct GemmArgs {
     int M, N, K;
@@ -53,6 +60,9 @@ template <int R, bool KM> struct Slab {
     static constexpr int KG = NV / 4;           // !KM: k per thread
     float v[NV];
 
+    // Loads with clamped in-bounds addresses and a select afterwards, so that
+    // no load sits behind a branch (all of a thread's loads issue before the
+    // first wait).  rmax, kmax >= 1.
     __device__ __forceinline__ void load(const float *P, int64_t ld, int r0, int rmax, int k0, int kmax,
                                          bool vec)
     {
@@ -60,37 +70,42 @@ template <int R, bool KM> struct Slab {
         if (KM) {
             const int row = r0 + t / TPR;
             const int k = k0 + (t % TPR) * SEG;
-            const float *p = P + (int64_t)row * ld + k;
-            if (row < rmax) {
-                if (vec && k + SEG <= kmax) {
+            const bool rin = row < rmax;
+            const float *p = P + (int64_t)min(row, rmax - 1) * ld;
+            if (vec && rin && k + SEG <= kmax) {
 #pragma unroll
-                    for (int e = 0; e < SEG; e += 4) {
-                        floatx4 x = *(const floatx4 *)(p + e);
-                        v[e] = x[0]; v[e + 1] = x[1]; v[e + 2] = x[2]; v[e + 3] = x[3];
-                    }
-                } else {
-#pragma unroll
-                    for (int e = 0; e < SEG; ++e) v[e] = (k + e < kmax) ? p[e] : 0.0f;
+                for (int e = 0; e < SEG; e += 4) {
+                    floatx4 x = *(const floatx4 *)(p + k + e);
+                    v[e] = x[0]; v[e + 1] = x[1]; v[e + 2] = x[2]; v[e + 3] = x[3];
                 }
             } else {
 #pragma unroll
-                for (int e = 0; e < SEG; ++e) v[e] = 0.0f;
+                for (int e = 0; e < SEG; ++e) v[e] = p[min(k + e, kmax - 1)];
+#pragma unroll
+                for (int e = 0; e < SEG; ++e)
+                    if (!(rin && k + e < kmax)) v[e] = 0.0f;
             }
         } else {
             const int rg = r0 + (t % (R / 4)) * 4;
             const int kb = k0 + (t / (R / 4)) * KG;
+            if (vec && rg + 4 <= rmax && kb + KG <= kmax) {
 #pragma unroll
-            for (int kk = 0; kk < KG; ++kk) {
-                const int k = kb + kk;
-                const float *p = P + (int64_t)k * ld + rg;
-                if (k < kmax && vec && rg + 4 <= rmax) {
-                    floatx4 x = *(const floatx4 *)p;
+                for (int kk = 0; kk < KG; ++kk) {
+                    floatx4 x = *(const floatx4 *)(P + (int64_t)(kb + kk) * ld + rg);
 #pragma unroll
                     for (int rr = 0; rr < 4; ++rr) v[kk * 4 + rr] = x[rr];
-                } else {
-#pragma unroll
-                    for (int rr = 0; rr < 4; ++rr) v[kk * 4 + rr] = (k < kmax && rg + rr < rmax) ? p[rr] : 0.0f;
                 }
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < KG; ++kk)
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr)
+                        v[kk * 4 + rr] = P[(int64_t)min(kb + kk, kmax - 1) * ld + min(rg + rr, rmax - 1)];
+#pragma unroll
+                for (int kk = 0; kk < KG; ++kk)
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr)
+                        if (!(kb + kk < kmax && rg + rr < rmax)) v[kk * 4 + rr] = 0.0f;
             }
         }
     }
@@ -115,8 +130,7 @@ template <int PREC, int R> struct Tile {
     static constexpr bool F32 = (PREC == GEMM_F32);
     static constexpr int LD = F32 ? (R + 4) : (BK + 8);
     static constexpr int SZ = F32 ? BK * LD : R * LD;
-    alignas(16) T hi[SZ];
-    alignas(16) T lo[PREC == GEMM_F16X3 ? SZ : 8];
+    alignas(16) T p[pieces<PREC>()][SZ];
 
     __device__ __forceinline__ int idx(int row, int k) const { return F32 ? k * LD + row : row * LD + k; }
 };
@@ -138,41 +152,45 @@ __device__ __forceinline__ void store_slab(Tile<PREC, R> &t, const Slab<R, KM> &
     if constexpr (PREC == GEMM_F32) {
         if constexpr (KM) {
 #pragma unroll
-            for (int e = 0; e < S::NV; ++e) t.hi[t.idx(s.row_of(e), s.k_of(e))] = s.v[e] * scale;
+            for (int e = 0; e < S::NV; ++e) t.p[0][t.idx(s.row_of(e), s.k_of(e))] = s.v[e] * scale;
         } else {
 #pragma unroll
             for (int kk = 0; kk < S::KG; ++kk) {
                 floatx4 x;
 #pragma unroll
                 for (int rr = 0; rr < 4; ++rr) x[rr] = s.v[kk * 4 + rr] * scale;
-                *(floatx4 *)&t.hi[t.idx(s.row_of(kk * 4), s.k_of(kk * 4))] = x;
+                *(floatx4 *)&t.p[0][t.idx(s.row_of(kk * 4), s.k_of(kk * 4))] = x;
             }
         }
     } else {
         constexpr int W = KM ? 8 : S::KG;      // elements per store
         constexpr int NST = S::NV / W;          // stores per array
+        constexpr int NP = pieces<PREC>();
         typedef typename Vec<T, W>::type VT;
 #pragma unroll
         for (int q = 0; q < NST; ++q) {
-            VT hv, lv;
-            int row, k;
+            VT pv[NP];
+            int row = 0, k = 0;
 #pragma unroll
             for (int e = 0; e < W; ++e) {
                 // KM: values q*W .. q*W+W-1 are consecutive k of one row.
                 // M-major: value kk*4+rr is row rr, k kk; store q = row rr.
                 const int idx = KM ? q * W + e : e * 4 + q;
                 float x = s.v[idx] * scale;
-                if constexpr (PREC == GEMM_F16X3) {
-                    _Float16 h = (_Float16)x;
-                    hv[e] = h;
-                    lv[e] = (_Float16)((x - (float)h) * 2048.0f);
+                if constexpr (NP == 1) {
+                    pv[0][e] = (T)x;
                 } else {
-                    hv[e] = (T)x;
+                    const _Float16 h = (_Float16)x;
+                    const float r = x - (float)h;  // exact
+                    const _Float16 m = (_Float16)(r * 2048.0f);
+                    pv[0][e] = h;
+                    pv[1][e] = m;
+                    if constexpr (NP == 3) pv[2][e] = (_Float16)((r - (float)m * 0x1p-11f) * 0x1p22f);
                 }
                 if (e == 0) { row = s.row_of(idx); k = s.k_of(idx); }
             }
-            *(VT *)&t.hi[t.idx(row, k)] = hv;
-            if constexpr (PREC == GEMM_F16X3) *(VT *)&t.lo[t.idx(row, k)] = lv;
+#pragma unroll
+            for (int c = 0; c < NP; ++c) *(VT *)&t.p[c][t.idx(row, k)] = pv[c];
         }
     }
 }
@@ -182,6 +200,7 @@ template <> struct Frag<GEMM_F32> { typedef float T; };
 template <> struct Frag<GEMM_F16> { typedef half8_t T; };
 template <> struct Frag<GEMM_BF16> { typedef bf16x8_t T; };
 template <> struct Frag<GEMM_F16X3> { typedef half8_t T; };
+template <> struct Frag<GEMM_F16X6> { typedef half8_t T; };
 
 template <int PREC>
 __device__ __forceinline__ floatx16 mfma(typename Frag<PREC>::T a, typename Frag<PREC>::T b, floatx16 c)
@@ -211,7 +230,8 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
 {
     constexpr int WM = BM / 2, WN = BN / 2;     // wave tile
     constexpr int FM = WM / 32, FN = WN / 32;   // 32x32 MFMA blocks per wave
-    constexpr bool X3 = (PREC == GEMM_F16X3);
+    constexpr int NP = pieces<PREC>();          // FP16 pieces per value
+    constexpr int NG = NP;                       // accumulator groups
     constexpr int KSTEPS = (PREC == GEMM_F32) ? BK / 2 : BK / 16;
 
     // Double-buffered operand tiles: one barrier per BK slab.
@@ -246,24 +266,32 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
     // The MFMA computes the transposed tile (B operand rows x A operand rows),
     // so that lane l%32 walks the rows of C: stores of column-major C are
     // then 128-byte contiguous per half-wave.
-    floatx16 acc[FM][FN], accl[X3 ? FM : 1][X3 ? FN : 1];
+    floatx16 acc[NG][FM][FN];
     double acc64[CH64 ? FM : 1][CH64 ? FN : 1][CH64 ? 16 : 1];
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int q = 0; q < NG; ++q)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-            if constexpr (X3) {
+            for (int j = 0; j < FN; ++j)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) accl[i][j][r] = 0.0f;
-            }
-            if constexpr (CH64) {
+                for (int r = 0; r < 16; ++r) acc[q][i][j][r] = 0.0f;
+    if constexpr (CH64) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc64[i][j][r] = 0.0;
-            }
-        }
+    }
 
+    // Value of accumulator element (i, j, r): groups combined in FP64.
+    auto combined = [&](int i, int j, int r) -> double {
+        double v = (double)acc[0][i][j][r];
+        if constexpr (NG > 1) v += (double)acc[1][i][j][r] * 0x1p-11;
+        if constexpr (NG > 2) v += (double)acc[2][i][j][r] * 0x1p-22;
+        return v;
+    };
     auto fold = [&]() {
         if constexpr (CH64) {
 #pragma unroll
@@ -272,10 +300,9 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
                 for (int j = 0; j < FN; ++j)
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        double v = (double)acc[i][j][r];
-                        if constexpr (X3) { v += (double)accl[i][j][r] * 0x1p-11; accl[i][j][r] = 0.0f; }
-                        acc64[i][j][r] += v;
-                        acc[i][j][r] = 0.0f;
+                        acc64[i][j][r] += combined(i, j, r);
+#pragma unroll
+                        for (int q = 0; q < NG; ++q) acc[q][i][j][r] = 0.0f;
                     }
         }
     };
@@ -298,34 +325,26 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
         }
 #pragma unroll
         for (int ks = 0; ks < KSTEPS; ++ks) {
-            typename Frag<PREC>::T fa[FM], fb[FN];
-            typename Frag<PREC>::T fal[X3 ? FM : 1], fbl[X3 ? FN : 1];
+            typename Frag<PREC>::T fa[NP][FM], fb[NP][FN];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) {
-                fa[i] = frag<PREC, BM>(As[cur].hi, wm * WM + 32 * i, ks);
-                if constexpr (X3) fal[i] = frag<PREC, BM>(As[cur].lo, wm * WM + 32 * i, ks);
-            }
+            for (int c = 0; c < NP; ++c) {
 #pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                fb[j] = frag<PREC, BN>(Bs[cur].hi, wn * WN + 32 * j, ks);
-                if constexpr (X3) fbl[j] = frag<PREC, BN>(Bs[cur].lo, wn * WN + 32 * j, ks);
+                for (int i = 0; i < FM; ++i) fa[c][i] = frag<PREC, BM>(As[cur].p[c], wm * WM + 32 * i, ks);
+#pragma unroll
+                for (int j = 0; j < FN; ++j) fb[c][j] = frag<PREC, BN>(Bs[cur].p[c], wn * WN + 32 * j, ks);
             }
-            // Grouped by product so that updates of one accumulator are
+            // Group g gets products p_a q_b with a + b = g; the loops run over
+            // the blocks innermost so that updates of one accumulator are
             // separated by independent MFMAs.
 #pragma unroll
-            for (int i = 0; i < FM; ++i)
+            for (int gq = 0; gq < NG; ++gq)
 #pragma unroll
-                for (int j = 0; j < FN; ++j) acc[i][j] = mfma<PREC>(fb[j], fa[i], acc[i][j]);
-            if constexpr (X3) {
+                for (int a = 0; a <= gq; ++a)
 #pragma unroll
-                for (int i = 0; i < FM; ++i)
+                    for (int i = 0; i < FM; ++i)
 #pragma unroll
-                    for (int j = 0; j < FN; ++j) accl[i][j] = mfma<PREC>(fbl[j], fa[i], accl[i][j]);
-#pragma unroll
-                for (int i = 0; i < FM; ++i)
-#pragma unroll
-                    for (int j = 0; j < FN; ++j) accl[i][j] = mfma<PREC>(fb[j], fal[i], accl[i][j]);
-            }
+                        for (int j = 0; j < FN; ++j)
+                            acc[gq][i][j] = mfma<PREC>(fb[gq - a][j], fa[a][i], acc[gq][i][j]);
         }
         if (CH64 && ((it + 1) % g.chunk_tiles == 0 || it + 1 == ntiles)) fold();
         if (more) {
@@ -351,8 +370,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
                 const float cs = oscale_a * ((g.b_exp) ? ldexpf(1.0f, -g.b_exp[col]) : 1.0f);
                 double v;
                 if constexpr (CH64) v = acc64[i][j][r];
-                else if constexpr (X3) v = (double)acc[i][j][r] + (double)accl[i][j][r] * 0x1p-11;
-                else v = (double)acc[i][j][r];
+                else v = combined(i, j, r);
                 v *= (double)cs;
                 if (g.partial) {
                     const int64_t off = ((int64_t)split * g.N + col) * g.M + row;
@@ -415,9 +433,13 @@ void launch_one(hipStream_t s, const GemmArgs &g, dim3 grid)
 template <int PREC, bool AKM, bool CH64>
 void launch_tile(hipStream_t s, const GemmArgs &g, int nsplit, bool small)
 {
-    if (small) {
-        dim3 grid((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
-        launch_one<PREC, 64, 64, AKM, CH64>(s, g, grid);
+    const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
+    if constexpr (PREC == GEMM_F16X6) {
+        // Three accumulator groups: a 128x128 tile would spill, so the
+        // three-piece form always runs 64x64 tiles.
+        launch_one<PREC, 64, 64, AKM, CH64>(s, g, g64);
+    } else if (small) {
+        launch_one<PREC, 64, 64, AKM, CH64>(s, g, g64);
     } else {
         dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
         launch_one<PREC, 128, 128, AKM, CH64>(s, g, grid);
@@ -432,6 +454,7 @@ int dispatch(hipStream_t s, int prec, const GemmArgs &g, int nsplit, bool small)
     case GEMM_F16: launch_tile<GEMM_F16, AKM, CH64>(s, g, nsplit, small); break;
     case GEMM_BF16: launch_tile<GEMM_BF16, AKM, CH64>(s, g, nsplit, small); break;
     case GEMM_F16X3: launch_tile<GEMM_F16X3, AKM, CH64>(s, g, nsplit, small); break;
+    case GEMM_F16X6: launch_tile<GEMM_F16X6, AKM, CH64>(s, g, nsplit, small); break;
     default: return LATER_ERR_OPTION;
     }
     HIP_OK(hipGetLastError());

@@ -37,16 +37,24 @@ __global__ __launch_bounds__(kThreads) void mgs_block_kernel(int m, int w, float
     const int lane = threadIdx.x & 63;
     float *Ab = A + r0;
 
+    // Loads from clamped in-bounds addresses, selected afterwards, so that all
+    // of them issue before the first wait.
     float a[kColsPerWave][kRowsPerLane];
 #pragma unroll
     for (int c = 0; c < kColsPerWave; ++c) {
         const int j = wave + kWaves * c;
+        const int jc = min(j, w - 1);
 #pragma unroll
         for (int t = 0; t < kRowsPerLane; ++t) {
             const int row = lane + 64 * t;
-            a[c][t] = (j < w && row < mm) ? Ab[row + (int64_t)j * lda] : 0.0f;
+            a[c][t] = Ab[min(row, mm - 1) + (int64_t)jc * lda];
         }
     }
+#pragma unroll
+    for (int c = 0; c < kColsPerWave; ++c)
+#pragma unroll
+        for (int t = 0; t < kRowsPerLane; ++t)
+            if (!(wave + kWaves * c < w && lane + 64 * t < mm)) a[c][t] = 0.0f;
     for (int i = threadIdx.x; i < kCols * (kCols + 1); i += kThreads) (&Rs[0][0])[i] = 0.0f;
     __syncthreads();
 
@@ -127,9 +135,13 @@ __global__ __launch_bounds__(256) void tsqr_apply_kernel(int m, int w, float *A,
     __syncthreads();
     float *Ab = A + r0;
     for (int row = threadIdx.x; row < mm; row += 256) {
+        // Unconditional loads (clamped column) so all of them are in flight at once.
         float v[kCols];
 #pragma unroll
-        for (int k = 0; k < kCols; ++k) v[k] = (k < w) ? Ab[row + (int64_t)k * lda] : 0.0f;
+        for (int k = 0; k < kCols; ++k) v[k] = Ab[row + (int64_t)min(k, w - 1) * lda];
+#pragma unroll
+        for (int k = 0; k < kCols; ++k)
+            if (k >= w) v[k] = 0.0f;
 #pragma unroll 4
         for (int j = 0; j < w; ++j) {
             Acc s = 0;

@@ -28,8 +28,10 @@ constexpr int kQExp = 14;          // |Q| <= 1: Q scaled by 2^14 before 16-bit c
 
 int split_n(int n) { return 32 * ((n + 63) / 64); }
 
-bool prec_ok(int p) { return p >= LATER_GEMM_F32 && p <= LATER_GEMM_F16X3; }
-bool scaled(int p) { return p == LATER_GEMM_F16X3; }
+bool prec_ok(int p) { return p >= LATER_GEMM_F32 && p <= LATER_GEMM_F16X6; }
+bool scaled(int p) { return p == LATER_GEMM_F16X3 || p == LATER_GEMM_F16X6; }
+// F16X6 is a projection precision; its updates use the hi/lo pair.
+int update_prec(int p) { return p == LATER_GEMM_F16X6 ? LATER_GEMM_F16X3 : p; }
 
 struct Layout {
     int64_t header, exps, panel, gemm, snode, r2, tcopy, total;
@@ -55,7 +57,7 @@ Layout layout(int m, int n, const later_opts &o)
     L.panel = panel_work_floats(m, 32);
     int64_t g = 0;
     walk_gemm(m, n, o.nmin, o.chunk, &g);
-    if (o.reorth & LATER_REORTH_PASS) walk_gemm(m, n, o.nmin, o.chunk, &g);
+    if (o.reorth & LATER_REORTH_PASS) walk_gemm(m, n, o.nmin, o.chunk2 > 0 ? o.chunk2 : o.chunk, &g);
     L.gemm = g;
     const int n1 = split_n(n);
     L.snode = ((o.reorth & LATER_REORTH_NODE) && n > 32) ? (int64_t)n1 * (n - n1) : 0;
@@ -76,6 +78,7 @@ struct Run {
     int launches;
     int depth;
     bool node_reorth;  // block CGS2 at the nodes of this pass
+    int chunk;         // FP64 fold chunk of this pass
 };
 
 __global__ void zero_lower_kernel(int n, float *R, int64_t ldr)
@@ -125,7 +128,7 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     const int n1 = split_n(n), n2 = n - n1;
     const bool inner = (n <= r.o.nmin);
     const int p = inner ? LATER_GEMM_F32 : prec;
-    const int chunk = inner ? 0 : r.o.chunk;
+    const int chunk = inner ? 0 : r.chunk;
     const bool f64 = chunk > 0;
     int rc = qr_rec(r, m, n1, A, lda, R, ldr, prec, depth + 1);
     if (rc) return rc;
@@ -145,7 +148,7 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     if (scaled(p)) {
         if ((rc = gemm_colexp(r.s, n1, n2, R12, ldr, r.exps, &r.launches))) return rc;
     }
-    rc = gemm_update(r.s, p, m, n1, n2, A, lda, R12, ldr, A2, lda, f64, bexp, aexp, &r.launches);
+    rc = gemm_update(r.s, update_prec(p), m, n1, n2, A, lda, R12, ldr, A2, lda, f64, bexp, aexp, &r.launches);
     if (rc) return rc;
     if (r.node_reorth) {
         // S = Q1^T A2; A2 -= Q1 S; R12 += S
@@ -159,7 +162,7 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
         if (scaled(p)) {
             if ((rc = gemm_colexp(r.s, n1, n2, S, n1, r.exps, &r.launches))) return rc;
         }
-        rc = gemm_update(r.s, p, m, n1, n2, A, lda, S, n1, A2, lda, f64, bexp, aexp, &r.launches);
+        rc = gemm_update(r.s, update_prec(p), m, n1, n2, A, lda, S, n1, A2, lda, f64, bexp, aexp, &r.launches);
         if (rc) return rc;
         hipLaunchKernelGGL(add_kernel, grid1((int64_t)n1 * n2), dim3(256), 0, r.s, n1, n2, S, (int64_t)n1, R12,
                            ldr);
@@ -186,6 +189,7 @@ LATER_API void later_opts_default(later_opts *o)
     o->reorth = LATER_REORTH_NONE;
     o->nmin = 128;
     o->chunk = 0;
+    o->chunk2 = 0;
 }
 
 LATER_API void later_opts_fp32(later_opts *o)
@@ -226,7 +230,8 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     int rc = LATER_OK;
     if (m < 1 || n < 1 || m < n || lda < m || ldr < n || !A || !R) rc = LATER_ERR_SHAPE;
     else if (!prec_ok(o.gemm) || !prec_ok(o.gemm_pass2) || o.panel < 0 || o.panel > 1 || o.reorth < 0 ||
-             o.reorth > 3 || o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0)
+             o.reorth > 3 || o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0 || o.chunk2 < 0 ||
+             (o.chunk2 % 32) != 0)
         rc = LATER_ERR_OPTION;
     Layout L = layout(m > 0 ? m : 1, n > 0 ? n : 1, o);
     // Everything lives in `work` when it is large enough; otherwise the
@@ -268,6 +273,7 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     r.launches = 0;
     r.depth = 0;
     r.node_reorth = (o.reorth & LATER_REORTH_NODE) != 0;
+    r.chunk = o.chunk;
 
     hipLaunchKernelGGL(status_clear_kernel, dim3(1), dim3(64), 0, r.s, r.status);
     hipLaunchKernelGGL(zero_lower_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R, (int64_t)ldr);
@@ -279,6 +285,7 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
         hipLaunchKernelGGL(fill_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, (int64_t)n * n, R2, 0.0f);
         ++r.launches;
         r.node_reorth = false;  // Q has condition number ~1: one projection per node suffices
+        r.chunk = o.chunk2 > 0 ? o.chunk2 : o.chunk;
         rc = qr_rec(r, m, n, A, lda, R2, n, o.gemm_pass2, 0);
         if (!rc) {
             hipLaunchKernelGGL(copy_upper_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R, (int64_t)ldr, T,
@@ -293,7 +300,7 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
                 bexp = r.exps;
             }
             if (!rc)
-                rc = gemm_nn(r.s, p2, n, n, n, 1.0f, R2, n, T, n, R, ldr, o.chunk > 0, bexp,
+                rc = gemm_nn(r.s, update_prec(p2), n, n, n, 1.0f, R2, n, T, n, R, ldr, o.chunk > 0, bexp,
                              scaled(p2) ? kQExp : 0, true, &r.launches);
         }
     }

@@ -58,7 +58,7 @@ def _round(X, prec):
     return t.numpy().astype(np.float64)
 
 
-PRECS = [(0, "f32"), (1, "f16"), (2, "bf16"), (3, "f16x3")]
+PRECS = [(0, "f32"), (1, "f16"), (2, "bf16"), (3, "f16x3"), (4, "f16x6")]
 
 
 @pytest.mark.parametrize("m,n1,n2", [(4096, 128, 128), (1000, 100, 36), (16384, 32, 32), (300, 7, 5),
@@ -72,9 +72,11 @@ def test_gemm_proj(later, oracle, m, n1, n2, prec, name, chunk):
     later.gemm_proj(_dev(later, Q), _dev(later, B), C, prec, chunk)
     _sync()
     Cg = _np(C).astype(np.float64)
-    if prec == 3:
+    if prec in (3, 4):
+        # Split products of the FP32 operands: hi/lo (2^-22) or three pieces
+        # (2^-33), FP32 accumulation (folded into FP64 every chunk rows).
         ref = Q.astype(np.float64).T @ B.astype(np.float64)
-        tol = 4e-6
+        tol = 4e-6 if prec == 3 else 4 * U32 * np.sqrt(chunk if chunk else m)
     else:
         ref = _round(Q, prec).T @ _round(B, prec)
         tol = 4 * U32 * np.sqrt(m)
@@ -94,9 +96,9 @@ def test_gemm_update(later, oracle, m, n1, n2, prec, name):
     later.gemm_update(_dev(later, Q), _dev(later, C), Bd, prec)
     _sync()
     Bg = _np(Bd).astype(np.float64)
-    if prec == 3:
+    if prec in (3, 4):
         ref = B.astype(np.float64) - Q.astype(np.float64) @ C.astype(np.float64)
-        tol = 4e-6
+        tol = 4e-6 if prec == 3 else 4 * U32 * np.sqrt(n1)
     else:
         ref = B.astype(np.float64) - _round(Q, prec) @ _round(C, prec)
         tol = 4 * U32 * np.sqrt(n1)
@@ -280,3 +282,26 @@ def test_check_metrics_match_oracle(later, oracle, m, n):
     b_g, b_o = later.check_backward(Ad, Qd, Rd), oracle.backward(A, Q, R)
     assert abs(o_g - o_o) < 1e-9 * max(1.0, o_o) + 1e-12
     assert abs(b_g - b_o) < 1e-6 * b_o + 1e-15
+
+
+@pytest.mark.parametrize("form", ["reference", "mixed"])
+def test_plan_replays_match_eager(later, oracle, form):
+    """A graph-captured plan replayed several times, with eager work and
+    synchronisations in between, gives the eager call's bits every time."""
+    import torch
+    m, n = 4096, 1024
+    A = oracle.gen_uniform(m, n, 0, 0)
+    Qe, Re, _ = _run(later, A, form)
+    o = later.opts(form)
+    Ad = _dev(later, A)
+    R = later.colmajor(n, n)
+    plan = later.Plan(Ad, R, o)
+    assert plan.kernels > 10
+    src = _dev(later, A)
+    for rep in range(3):
+        Ad.copy_(src)
+        plan.execute()
+        _sync()
+        assert np.array_equal(_np(Ad), Qe) and np.array_equal(_np(R), Re), rep
+        _run(later, oracle.gen_uniform(512, 64, rep, 1), form)  # eager work in between
+    plan.close()
