@@ -88,6 +88,45 @@ def profile_step(later, torch, A0, form, work, o):
     return later.prof_end()
 
 
+def vendor_sgeqrf(torch, A0, reps=2):
+    """rocSOLVER sgeqrf on the same matrix, timed like the reference times
+    cuSOLVER SGEQRF beside rgsqrf (test/test_qr.cu:184-206): R and the
+    Householder vectors only, no explicit Q.  Returns ms or None."""
+    import ctypes
+    try:
+        rb = ctypes.CDLL("/opt/rocm/lib/librocblas.so")
+        rs = ctypes.CDLL("/opt/rocm/lib/librocsolver.so")
+    except OSError as e:
+        log(f"[bench] rocSOLVER unavailable: {e}")
+        return None
+    m, n = A0.shape
+    h = ctypes.c_void_p()
+    if rb.rocblas_create_handle(ctypes.byref(h)) != 0:
+        return None
+    s = torch.cuda.current_stream()
+    rb.rocblas_set_stream(h, ctypes.c_void_p(s.cuda_stream))
+    A = A0.clone().t().contiguous().t()
+    tau = torch.empty(min(m, n), device="cuda", dtype=torch.float32)
+    rs.rocsolver_sgeqrf.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.c_void_p]
+    best = None
+    for _ in range(reps):
+        A.copy_(A0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(s)
+        rc = rs.rocsolver_sgeqrf(h, m, n, A.data_ptr(), A.stride(1), tau.data_ptr())
+        e1.record(s)
+        torch.cuda.synchronize()
+        if rc != 0:
+            log(f"[bench] rocsolver_sgeqrf status {rc}")
+            return None
+        t = e0.elapsed_time(e1)
+        best = t if best is None else min(best, t)
+    rb.rocblas_destroy_handle(h)
+    return best
+
+
 def cpu_baseline(n_sample: int):
     from oracle import oracle as O
     import numpy as np
@@ -113,6 +152,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the rocSOLVER sgeqrf baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this QR path runs on one GPU; multi-GPU row sharding is described "
@@ -158,6 +198,10 @@ def main():
                 "kernel": "gemm_kernel (node projection + update, FP16 hi/lo MFMA)",
                 "launches": launches, "kernel_ms": gemm_ms}
 
+    vend = None if args.no_vendor else vendor_sgeqrf(torch, A0)
+    if vend:
+        log(f"[bench] rocSOLVER sgeqrf: {vend:.2f} ms  {later.flops_ref(m, n) / vend / 1e9:.1f} TFLOP/s(ref)")
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline sample n={args.cpu_sample}")
@@ -187,6 +231,9 @@ def main():
                        "orth": mixed.get("orth"), "orth_over_n": mixed.get("orth_over_n"),
                        "backward": mixed.get("backward")},
         "speedup_vs_fp32": fp32["ms"] / mixed["ms"],
+        "vendor_sgeqrf": ({"ms": vend, "tflops_ref": later.flops_ref(m, n) / vend / 1e9,
+                           "what": "rocSOLVER sgeqrf (R + reflectors, no explicit Q), same input"}
+                          if vend else None),
         "gates_met": (mixed.get("orth", 1.0) <= 1e-5 and mixed.get("backward", 1.0) <= 1e-6),
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -70,7 +70,9 @@ enum {
 /* Panel (leaf) algorithm. */
 enum {
     LATER_PANEL_MGS = 0,    /* reference form: TSQR tree of block MGS, FP32 sums */
-    LATER_PANEL_MGS64 = 1   /* same tree, FP64 column norms and dot products */
+    LATER_PANEL_MGS64 = 1,  /* same tree, FP64 column norms and dot products */
+    LATER_PANEL_CHOLQR2 = 2 /* Cholesky QR twice with FP64 Gram factors: three
+                               grid-wide launches, no sequential column loop */
 };
 
 /* Re-orthogonalisation (bit flags; NODE|PASS = 3 combines both). */
@@ -134,6 +136,23 @@ int later_plan_execute(void *plan, void *stream);
 int later_plan_kernels(void *plan);
 int later_plan_destroy(void *plan);
 
+/* Batched QR of `count` independent matrices A_i (m_i x n_i, lda_i, device
+ * pointers) into R_i (n_i x n_i, ldr_i), no counterpart in the reference.
+ * later_batch_workspace gives the FP32 workspace; later_batch_plan_create
+ * builds the grouped task tables (matrices bucketed by ceil(n/32), one
+ * grouped launch per step of the shared recursion tree, one stream per
+ * bucket) and captures them into a HIP graph -- it allocates its tables with
+ * hipMalloc once; later_batch_plan_execute replays the graph on `stream`.
+ * Options as for later_rgsqrf_ex except LATER_REORTH_NODE.  Matrix i's
+ * status word (bit 0: breakdown) is the first int of its workspace block. */
+int later_batch_workspace(int count, const int *m, const int *n, const later_opts *opts, int64_t *lwork);
+int later_batch_plan_create(int count, const int *m, const int *n, float *const *A, const int *lda,
+                            float *const *R, const int *ldr, float *work, int64_t lwork,
+                            const later_opts *opts, void **plan);
+int later_batch_plan_execute(void *plan, void *stream);
+int later_batch_plan_launches(void *plan);
+int later_batch_plan_destroy(void *plan);
+
 /* Synchronises ctxt.stream and reads the device status word that the last
  * later_rgsqrf_ex / later_caqr_panel call on this `work` left: bit 0 set
  * means a zero or non-finite column norm (LATER_ERR_BREAKDOWN); the
@@ -141,7 +160,9 @@ int later_plan_destroy(void *plan);
 int later_read_status(laterCtxt ctxt, const float *work, int *device_status);
 
 /* Panel of the reference alone: mgs_caqr_panel_256x32 (QR/panel.cu:65-134)
- * generalised to w <= 32 columns.  A (m x w) becomes Q, R (w x w). */
+ * generalised to w <= 32 columns.  A (m x w) becomes Q, R (w x w).
+ * fp64_sums: 0 the reference's FP32 block MGS, 1 block MGS with FP64 sums,
+ * 2 Cholesky QR twice with FP64 Gram factors (LATER_PANEL_CHOLQR2). */
 int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, float *R, int ldr,
                      float *work, int64_t lwork, int fp64_sums);
 

@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(_HERE, "liblater_amd.so")
 # Status codes (include/later.h)
 OK, ERR_SHAPE, ERR_WORKSPACE, ERR_OPTION, ERR_DEVICE, ERR_BREAKDOWN, ERR_ALIAS, ERR_BUDGET = range(8)
 GEMM_F32, GEMM_F16, GEMM_BF16, GEMM_F16X3, GEMM_F16X6 = range(5)
-PANEL_MGS, PANEL_MGS64 = range(2)
+PANEL_MGS, PANEL_MGS64, PANEL_CHOLQR2 = range(3)
 REORTH_NONE, REORTH_NODE, REORTH_PASS = range(3)
 
 _STATUS_NAMES = {
@@ -97,6 +97,14 @@ _SIGNATURES = {
     "later_plan_execute": (_i, [_vp, _vp]),
     "later_plan_kernels": (_i, [_vp]),
     "later_plan_destroy": (_i, [_vp]),
+    "later_batch_workspace": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(later_opts),
+                                   ctypes.POINTER(_i64)]),
+    "later_batch_plan_create": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_vp),
+                                     ctypes.POINTER(_i), ctypes.POINTER(_vp), ctypes.POINTER(_i), _vp, _i64,
+                                     ctypes.POINTER(later_opts), ctypes.POINTER(_vp)]),
+    "later_batch_plan_execute": (_i, [_vp, _vp]),
+    "later_batch_plan_launches": (_i, [_vp]),
+    "later_batch_plan_destroy": (_i, [_vp]),
     "later_prof_begin": (None, []),
     "later_prof_end": (_i, [ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
 }
@@ -235,6 +243,53 @@ class Plan:
             pass
 
 
+class BatchPlan:
+    """Graph-captured batched QR of the column-major tensors As (in place) into Rs."""
+
+    def __init__(self, As, Rs, o: later_opts | None = None):
+        torch = _torch()
+        cnt = len(As)
+        I = _i * cnt
+        P = _vp * cnt
+        self.m = I(*[a.shape[0] for a in As])
+        self.n = I(*[a.shape[1] for a in As])
+        self.lda = I(*[_ld(a) for a in As])
+        self.ldr = I(*[_ld(r) for r in Rs])
+        self.A = P(*[a.data_ptr() for a in As])
+        self.R = P(*[r.data_ptr() for r in Rs])
+        lw = _i64()
+        _check(lib().later_batch_workspace(cnt, self.m, self.n, ctypes.byref(o) if o is not None else None,
+                                           ctypes.byref(lw)), "later_batch_workspace")
+        self.work = torch.empty(max(lw.value, 1), device=As[0].device, dtype=torch.float32)
+        self.keep = (As, Rs)
+        h = _vp()
+        _check(lib().later_batch_plan_create(cnt, self.m, self.n, self.A, self.lda, self.R, self.ldr,
+                                             self.work.data_ptr(), self.work.numel(),
+                                             ctypes.byref(o) if o is not None else None, ctypes.byref(h)),
+               "later_batch_plan_create")
+        self.h = h
+
+    @property
+    def launches(self) -> int:
+        return int(lib().later_batch_plan_launches(self.h))
+
+    def execute(self, stream=None):
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _check(lib().later_batch_plan_execute(self.h, ctypes.c_void_p(s.cuda_stream)), "later_batch_plan_execute")
+
+    def close(self):
+        if self.h:
+            lib().later_batch_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def rgsqrf_legacy(A, R, work, hwork, stream=None):
     """The reference-signature entry point (include/LATER.h:39)."""
     m, n = A.shape
@@ -249,14 +304,15 @@ def read_status(work, stream=None) -> int:
     return int(st.value)
 
 
-def caqr_panel(A, R=None, fp64: bool = False, work=None, stream=None):
-    """mgs_caqr_panel_256x32 (QR/panel.cu:65-134) for an m x w panel, w <= 32."""
+def caqr_panel(A, R=None, fp64=False, work=None, stream=None):
+    """mgs_caqr_panel_256x32 (QR/panel.cu:65-134) for an m x w panel, w <= 32.
+    fp64: False/0 FP32 block MGS, True/1 FP64 sums, 2 Cholesky QR twice."""
     torch = _torch()
     m, w = A.shape
     if R is None:
         R = colmajor(w, w, device=A.device)
     if work is None:
-        work = torch.empty(64 + max(1, 40 * m), device=A.device, dtype=torch.float32)
+        work = torch.empty(64 + max(600000, 40 * m), device=A.device, dtype=torch.float32)
     _check(lib().later_caqr_panel(ctxt(stream), m, w, A.data_ptr(), _ld(A), R.data_ptr(), _ld(R),
                                   work.data_ptr(), work.numel(), int(fp64)), "later_caqr_panel")
     return R

@@ -5,6 +5,37 @@
 enum { GEMM_F32 = LATER_GEMM_F32, GEMM_F16 = LATER_GEMM_F16, GEMM_BF16 = LATER_GEMM_BF16,
        GEMM_F16X3 = LATER_GEMM_F16X3, GEMM_F16X6 = LATER_GEMM_F16X6 };
 
+// One product of the tiled MFMA kernel (device-side argument block; the
+// grouped launches read an array of these).
+struct GemmArgs {
+    int M, N, K;
+    const float *A; int64_t lda;   // TN: K x M (col i of A = row i of op(A)); NN: M x K
+    const float *B; int64_t ldb;   // K x N
+    float *C; int64_t ldc;         // M x N
+    int a_exp;                     // op(A) scaled by 2^a_exp before the cast
+    const int *b_exp;              // column j of B scaled by 2^b_exp[j] (nullptr: none)
+    int kper;                      // K rows per split (multiple of BK)
+    int chunk_tiles;               // BK tiles per FP64 fold (0: no FP64 accumulation)
+    void *partial;                 // split-K partials (nsplit x M x N), nullptr: write C
+    int update;                    // 0: C = result; 1: C += alpha * result
+    float alpha;
+    int tri;                       // op(A), B upper triangular: skip zero tiles and K ranges
+};
+
+struct ColexpArgs {
+    int m;
+    const float *X;
+    int64_t ldx;
+    int *e;
+};
+
+// Grouped launches (64x64 tiles, no split-K): task b = (args index, tile row,
+// tile col, split) for the GEMM, (args index, column) for colexp.  proj
+// selects the K-major A operand (C = A^T B); otherwise A is M-major (C += alpha A B).
+int gemm_grouped(hipStream_t s, int prec, bool proj, bool ch64, const GemmArgs *args, const int4 *tasks,
+                 int ntasks);
+int colexp_grouped(hipStream_t s, const ColexpArgs *args, const int2 *tasks, int ntasks);
+
 // Floats of workspace the projection needs for its split-K partials.
 int64_t gemm_proj_work_floats(int m, int n1, int n2, int chunk);
 

@@ -35,20 +35,7 @@ template <> struct Lds<GEMM_F16X6> { typedef _Float16 T; };
 // group g holds sum_{i+j=g} p_i q_j.
 template <int PREC> constexpr int pieces() { return PREC == GEMM_F16X6 ? 3 : PREC == GEMM_F16X3 ? 2 : 1; }
 
-struct GemmArgs {
-    int M, N, K;
-    const float *A; int64_t lda;   // TN: K x M (col i of A = row i of op(A)); NN: M x K
-    const float *B; int64_t ldb;   // K x N
-    float *C; int64_t ldc;         // M x N
-    int a_exp;                     // op(A) scaled by 2^a_exp before the cast
-    const int *b_exp;              // column j of B scaled by 2^b_exp[j] (nullptr: none)
-    int kper;                      // K rows per split (multiple of BK)
-    int chunk_tiles;               // BK tiles per FP64 fold (0: no FP64 accumulation)
-    void *partial;                 // split-K partials (nsplit x M x N), nullptr: write C
-    int update;                    // 0: C = result; 1: C += alpha * result
-    float alpha;
-    int tri;                       // op(A), B upper triangular: skip zero tiles and K ranges
-};
+
 
 // Loads of one BK-deep slab of an R-row operand tile into registers.
 //   KM (rows contiguous along K): thread owns SEG consecutive k of one row.
@@ -226,7 +213,7 @@ __device__ __forceinline__ typename Frag<PREC>::T frag(const typename Lds<PREC>:
 }
 
 template <int PREC, int BM, int BN, bool AKM, bool CH64>
-__global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
+__device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const int tn, const int split)
 {
     constexpr int WM = BM / 2, WN = BN / 2;     // wave tile
     constexpr int FM = WM / 32, FN = WN / 32;   // 32x32 MFMA blocks per wave
@@ -238,7 +225,6 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
     __shared__ Tile<PREC, BM> As[2];
     __shared__ Tile<PREC, BN> Bs[2];
 
-    const int tm = blockIdx.x, tn = blockIdx.y, split = blockIdx.z;
     const int m0 = tm * BM, n0 = tn * BN;
     int kbeg = split * g.kper;
     int kend = min(g.K, kbeg + g.kper);
@@ -388,6 +374,22 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
     }
 }
 
+template <int PREC, int BM, int BN, bool AKM, bool CH64>
+__global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
+{
+    gemm_body<PREC, BM, BN, AKM, CH64>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Grouped launch: workgroup b computes tile (t.y, t.z) of split t.w of the
+// product described by args[t.x], t = tasks[b].
+template <int PREC, int BM, int BN, bool AKM, bool CH64>
+__global__ __launch_bounds__(NT) void gemm_grouped_kernel(const GemmArgs *args, const int4 *tasks)
+{
+    const int4 t = tasks[blockIdx.x];
+    const GemmArgs g = args[t.x];
+    gemm_body<PREC, BM, BN, AKM, CH64>(g, t.y, t.z, t.w);
+}
+
 // C = sum over splits (fixed order, FP64) of the partials.
 template <bool F64>
 __global__ void splitk_reduce_kernel(int M, int N, int nsplit, const void *partial, float *C, int64_t ldc)
@@ -421,6 +423,27 @@ __global__ void colexp_kernel(int m, int n, const float *X, int64_t ldx, int *e)
             ex = 15 - fe;
         }
         e[j] = ex;
+    }
+}
+
+// Grouped colexp: workgroup b handles column t.y of matrix args[t.x].
+__global__ void colexp_grouped_kernel(const ColexpArgs *args, const int2 *tasks)
+{
+    const int2 t = tasks[blockIdx.x];
+    const ColexpArgs a = args[t.x];
+    const float *x = a.X + (int64_t)t.y * a.ldx;
+    float mx = 0.0f;
+    for (int i = threadIdx.x; i < a.m; i += 64) mx = fmaxf(mx, fabsf(x[i]));
+#pragma unroll
+    for (int mask = 32; mask > 0; mask >>= 1) mx = fmaxf(mx, __shfl_xor(mx, mask, 64));
+    if (threadIdx.x == 0) {
+        int ex = 0;
+        if (mx > 0.0f && mx < INFINITY) {
+            int fe;
+            frexpf(mx, &fe);
+            ex = 15 - fe;
+        }
+        a.e[t.y] = ex;
     }
 }
 
@@ -461,7 +484,46 @@ int dispatch(hipStream_t s, int prec, const GemmArgs &g, int nsplit, bool small)
     return LATER_OK;
 }
 
+template <int PREC, bool AKM, bool CH64>
+void launch_grouped_t(hipStream_t s, const GemmArgs *args, const int4 *tasks, int ntasks)
+{
+    hipLaunchKernelGGL((gemm_grouped_kernel<PREC, 64, 64, AKM, CH64>), dim3(ntasks), dim3(NT), 0, s, args, tasks);
+}
+
+template <bool AKM, bool CH64>
+int launch_grouped_p(hipStream_t s, int prec, const GemmArgs *args, const int4 *tasks, int ntasks)
+{
+    switch (prec) {
+    case GEMM_F32: launch_grouped_t<GEMM_F32, AKM, CH64>(s, args, tasks, ntasks); break;
+    case GEMM_F16: launch_grouped_t<GEMM_F16, AKM, CH64>(s, args, tasks, ntasks); break;
+    case GEMM_BF16: launch_grouped_t<GEMM_BF16, AKM, CH64>(s, args, tasks, ntasks); break;
+    case GEMM_F16X3: launch_grouped_t<GEMM_F16X3, AKM, CH64>(s, args, tasks, ntasks); break;
+    case GEMM_F16X6: launch_grouped_t<GEMM_F16X6, AKM, CH64>(s, args, tasks, ntasks); break;
+    default: return LATER_ERR_OPTION;
+    }
+    HIP_OK(hipGetLastError());
+    return LATER_OK;
+}
+
 } // namespace
+
+int gemm_grouped(hipStream_t s, int prec, bool proj, bool ch64, const GemmArgs *args, const int4 *tasks,
+                 int ntasks)
+{
+    if (ntasks <= 0) return LATER_OK;
+    if (proj) return ch64 ? launch_grouped_p<true, true>(s, prec, args, tasks, ntasks)
+                          : launch_grouped_p<true, false>(s, prec, args, tasks, ntasks);
+    return ch64 ? launch_grouped_p<false, true>(s, prec, args, tasks, ntasks)
+                : launch_grouped_p<false, false>(s, prec, args, tasks, ntasks);
+}
+
+int colexp_grouped(hipStream_t s, const ColexpArgs *args, const int2 *tasks, int ntasks)
+{
+    if (ntasks <= 0) return LATER_OK;
+    hipLaunchKernelGGL(colexp_grouped_kernel, dim3(ntasks), dim3(64), 0, s, args, tasks);
+    HIP_OK(hipGetLastError());
+    return LATER_OK;
+}
 
 // Split plan of the projection, a function of (m, n1, n2, chunk) only, so the
 // summation order (and the result bits) never depends on the caller's

@@ -23,13 +23,12 @@ constexpr int kRowsPerLane = 5;           // up to 320 rows per block
 constexpr int kMaxRows = 64 * kRowsPerLane;
 
 template <typename Acc>
-__global__ __launch_bounds__(kThreads) void mgs_block_kernel(int m, int w, float *A, int lda, float *Rout,
-                                                             int ldr, int bh, int nblk, int *status)
+__device__ __forceinline__ void mgs_block_body(int m, int w, float *A, int64_t lda, float *Rout, int64_t ldr,
+                                               int bh, int nblk, int *status, const int b)
 {
     __shared__ float qs[2][kMaxRows];
     __shared__ float Rs[kCols][kCols + 1];
 
-    const int b = blockIdx.x;
     const int r0 = b * bh;
     // The last block takes every remaining row (a tail shorter than w is merged).
     const int mm = (b == nblk - 1) ? (m - r0) : bh;
@@ -120,11 +119,25 @@ __global__ __launch_bounds__(kThreads) void mgs_block_kernel(int m, int w, float
 // update never reads a value another thread writes (the reference relies on
 // an aliased batched Sgemm here, QR/panel.cu:97-102).
 template <typename Acc>
-__global__ __launch_bounds__(256) void tsqr_apply_kernel(int m, int w, float *A, int lda, const float *W, int ldw,
-                                                         int bh, int nblk)
+__global__ __launch_bounds__(kThreads) void mgs_block_kernel(int m, int w, float *A, int lda, float *Rout,
+                                                             int ldr, int bh, int nblk, int *status)
+{
+    mgs_block_body<Acc>(m, w, A, lda, Rout, ldr, bh, nblk, status, blockIdx.x);
+}
+
+template <typename Acc>
+__global__ __launch_bounds__(kThreads) void mgs_block_grouped_kernel(const PanelArgs *args, const int2 *tasks)
+{
+    const int2 t = tasks[blockIdx.x];
+    const PanelArgs a = args[t.x];
+    mgs_block_body<Acc>(a.m, a.w, a.A, a.lda, a.W, a.ldw, kPanelBlockRows, a.nblk, a.status, t.y);
+}
+
+template <typename Acc>
+__device__ __forceinline__ void tsqr_apply_body(int m, int w, float *A, int64_t lda, const float *W, int64_t ldw,
+                                                int bh, int nblk, const int b)
 {
     __shared__ float Ws[kCols][kCols];
-    const int b = blockIdx.x;
     const int r0 = b * bh;
     const int mm = (b == nblk - 1) ? (m - r0) : bh;
     const float *Wb = W + (int64_t)b * w;
@@ -153,6 +166,21 @@ __global__ __launch_bounds__(256) void tsqr_apply_kernel(int m, int w, float *A,
     }
 }
 
+template <typename Acc>
+__global__ __launch_bounds__(256) void tsqr_apply_kernel(int m, int w, float *A, int lda, const float *W, int ldw,
+                                                         int bh, int nblk)
+{
+    tsqr_apply_body<Acc>(m, w, A, lda, W, ldw, bh, nblk, blockIdx.x);
+}
+
+template <typename Acc>
+__global__ __launch_bounds__(256) void tsqr_apply_grouped_kernel(const PanelArgs *args, const int2 *tasks)
+{
+    const int2 t = tasks[blockIdx.x];
+    const PanelArgs a = args[t.x];
+    tsqr_apply_body<Acc>(a.m, a.w, a.A, a.lda, a.W, a.ldw, kPanelBlockRows, a.nblk, t.y);
+}
+
 struct Blocks {
     int nblk;
 };
@@ -166,6 +194,39 @@ Blocks plan_blocks(int m, int w)
 }
 
 } // namespace
+
+int panel_blocks(int m, int w) { return plan_blocks(m, w).nblk; }
+
+int mgs_grouped(hipStream_t s, bool fp64, const PanelArgs *args, const int2 *tasks, int ntasks)
+{
+    if (ntasks <= 0) return LATER_OK;
+    if (fp64) hipLaunchKernelGGL(mgs_block_grouped_kernel<double>, dim3(ntasks), dim3(kThreads), 0, s, args, tasks);
+    else hipLaunchKernelGGL(mgs_block_grouped_kernel<float>, dim3(ntasks), dim3(kThreads), 0, s, args, tasks);
+    HIP_OK(hipGetLastError());
+    return LATER_OK;
+}
+
+int apply_grouped(hipStream_t s, bool fp64, const PanelArgs *args, const int2 *tasks, int ntasks)
+{
+    if (ntasks <= 0) return LATER_OK;
+    if (fp64) hipLaunchKernelGGL(tsqr_apply_grouped_kernel<double>, dim3(ntasks), dim3(256), 0, s, args, tasks);
+    else hipLaunchKernelGGL(tsqr_apply_grouped_kernel<float>, dim3(ntasks), dim3(256), 0, s, args, tasks);
+    HIP_OK(hipGetLastError());
+    return LATER_OK;
+}
+
+int64_t panel_work_floats_any(int m)
+{
+    // A ragged leaf can be narrower than 32 columns; its tree can need a
+    // stack where the 32-wide tree has none (a short tail is merged only when
+    // it has fewer rows than the panel has columns).
+    int64_t mx = 0;
+    for (int w = 1; w <= kCols; ++w) {
+        const int64_t f = panel_work_floats(m, w);
+        if (f > mx) mx = f;
+    }
+    return mx;
+}
 
 int64_t panel_work_floats(int m, int w)
 {

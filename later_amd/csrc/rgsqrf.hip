@@ -54,7 +54,7 @@ Layout layout(int m, int n, const later_opts &o)
     Layout L;
     L.header = kHeaderFloats;
     L.exps = ((int64_t)n + 63) / 64 * 64;
-    L.panel = panel_work_floats(m, 32);
+    L.panel = (o.panel == LATER_PANEL_CHOLQR2) ? panel_chol_work_floats() : panel_work_floats_any(m);
     int64_t g = 0;
     walk_gemm(m, n, o.nmin, o.chunk, &g);
     if (o.reorth & LATER_REORTH_PASS) walk_gemm(m, n, o.nmin, o.chunk2 > 0 ? o.chunk2 : o.chunk, &g);
@@ -122,9 +122,13 @@ dim3 grid1(int64_t n) { return dim3((unsigned)ceil_div64(n, 256)); }
 int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, int prec, int depth)
 {
     if (depth > r.depth) r.depth = depth;
-    if (n <= 32)
+    if (n <= 32) {
+        if (r.o.panel == LATER_PANEL_CHOLQR2)
+            return panel_cholqr2(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n, r.status + 1, r.status,
+                                 &r.launches);
         return panel_tsqr(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n,
                           r.o.panel == LATER_PANEL_MGS64, r.status, &r.launches);
+    }
     const int n1 = split_n(n), n2 = n - n1;
     const bool inner = (n <= r.o.nmin);
     const int p = inner ? LATER_GEMM_F32 : prec;
@@ -229,7 +233,7 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     if (info) memset(info, 0, sizeof(*info));
     int rc = LATER_OK;
     if (m < 1 || n < 1 || m < n || lda < m || ldr < n || !A || !R) rc = LATER_ERR_SHAPE;
-    else if (!prec_ok(o.gemm) || !prec_ok(o.gemm_pass2) || o.panel < 0 || o.panel > 1 || o.reorth < 0 ||
+    else if (!prec_ok(o.gemm) || !prec_ok(o.gemm_pass2) || o.panel < 0 || o.panel > 2 || o.reorth < 0 ||
              o.reorth > 3 || o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0 || o.chunk2 < 0 ||
              (o.chunk2 % 32) != 0)
         rc = LATER_ERR_OPTION;
@@ -334,10 +338,14 @@ LATER_API int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, 
                                int64_t lwork, int fp64_sums)
 {
     if (w < 1 || w > 32 || m < w || lda < m || ldr < w || !A || !R) return LATER_ERR_SHAPE;
-    const int64_t need = kHeaderFloats + panel_work_floats(m, w);
+    if (fp64_sums < 0 || fp64_sums > 2) return LATER_ERR_OPTION;
+    const int64_t need = kHeaderFloats + (fp64_sums == 2 ? panel_chol_work_floats() : panel_work_floats(m, w));
     if (!work || lwork < need) return LATER_ERR_WORKSPACE;
     hipStream_t s = ctx_stream(ctxt);
     hipLaunchKernelGGL(status_clear_kernel, dim3(1), dim3(64), 0, s, (int *)work);
+    if (fp64_sums == 2)
+        return panel_cholqr2(s, m, w, A, lda, R, ldr, work + kHeaderFloats, need - kHeaderFloats, (int *)work + 1,
+                             (int *)work, nullptr);
     return panel_tsqr(s, m, w, A, lda, R, ldr, work + kHeaderFloats, need - kHeaderFloats, fp64_sums != 0,
                       (int *)work, nullptr);
 }

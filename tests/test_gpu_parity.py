@@ -110,7 +110,7 @@ def test_gemm_update(later, oracle, m, n1, n2, prec, name):
 # ---------------------------------------------------------------- panel
 @pytest.mark.parametrize("m,w", [(256, 32), (4096, 32), (16384, 32), (1000, 32), (287, 32), (300, 7),
                                  (64, 1), (70000, 32)])
-@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("fp64", [0, 1, 2])
 def test_caqr_panel(later, oracle, m, w, fp64):
     A = oracle.gen_uniform(m, w, 0, 0)
     Ad = _dev(later, A)
@@ -284,6 +284,15 @@ def test_check_metrics_match_oracle(later, oracle, m, n):
     assert abs(b_g - b_o) < 1e-6 * b_o + 1e-15
 
 
+@pytest.mark.parametrize("m,n,kind", [(2048, 2048, "normal"), (4096, 1024, "uniform")])
+def test_rgsqrf_mixed_cholqr2_panel(later, oracle, m, n, kind):
+    A = oracle.gen_normal(m, n, 0, 0) if kind == "normal" else oracle.gen_uniform(m, n, 0, 0)
+    Q, R, _ = _run(later, A, "mixed", panel=later.PANEL_CHOLQR2)
+    Qh = np.linalg.qr(A)[0]
+    assert oracle.orth(Q) < 6 * oracle.orth(Qh)
+    assert oracle.backward(A, Q, R) < 1e-6
+
+
 @pytest.mark.parametrize("form", ["reference", "mixed"])
 def test_plan_replays_match_eager(later, oracle, form):
     """A graph-captured plan replayed several times, with eager work and
@@ -305,3 +314,33 @@ def test_plan_replays_match_eager(later, oracle, form):
         assert np.array_equal(_np(Ad), Qe) and np.array_equal(_np(R), Re), rep
         _run(later, oracle.gen_uniform(512, 64, rep, 1), form)  # eager work in between
     plan.close()
+
+
+@pytest.mark.parametrize("m,w", [(4096, 32), (300, 5), (1000000, 32)])
+def test_cholqr2_panel_ill_conditioned(later, oracle, m, w):
+    """Cholesky QR twice on a panel with kappa = 1e6 (graded columns)."""
+    rng = np.random.default_rng(0)
+    U, _ = np.linalg.qr(rng.standard_normal((min(m, 20000), w)))
+    if m > 20000:
+        U = np.vstack([U] * (m // 20000) + [U[: m % 20000]])
+        U, _ = np.linalg.qr(U)
+    V, _ = np.linalg.qr(rng.standard_normal((w, w)))
+    s = 10.0 ** (-6 * np.arange(w) / max(w - 1, 1))
+    A = np.asfortranarray((U * s) @ V.T, dtype=np.float32)
+    Ad = _dev(later, A)
+    R = later.caqr_panel(Ad, fp64=2)
+    _sync()
+    Qg, Rg = _np(Ad), _np(R)
+    assert oracle.orth(Qg) < 2e-6 * np.sqrt(w)
+    assert oracle.backward(A, Qg, Rg) < 1e-6
+    assert np.array_equal(Rg, np.triu(Rg))
+
+
+def test_cholqr2_panel_breakdown(later, oracle):
+    import torch
+    A = oracle.gen_uniform(2000, 16, 0, 0)
+    A[:, 4] = A[:, 3]  # rank deficient
+    Ad = _dev(later, A)
+    work = torch.empty(64 + 600000, device="cuda")
+    later.caqr_panel(Ad, fp64=2, work=work)
+    assert later.read_status(work) & 1
