@@ -35,8 +35,8 @@ int64_t panel_work_floats_any(int m);
 int panel_tsqr(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr, float *work,
                int64_t lwork, bool fp64, int *status, int *launches);
 
-// Cholesky QR twice for an m x w panel (w <= 32): see panel_chol.hip.
-// counters: one int of zeroed device memory (reset after use).
+// Cholesky QR twice for an m x w panel (w <= 32) in three launches; *status
+// gets bit 0 set on a numerically rank-deficient or non-finite Gram pivot.
 int64_t panel_chol_work_floats();
 int panel_cholqr2(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr, float *work, int64_t lwork,
-                  int *counters, int *status, int *launches);
+                  int *status, int *launches);

@@ -1,288 +1,388 @@
 // Tall-skinny panel by Cholesky QR, twice (CholQR2), with FP64 Gram factors.
 //
 // An alternative to the block-MGS TSQR tree for the leaves of the accurate
-// forms.  It has no sequential loop over columns across the grid: for an
-// m x w panel (w <= 32)
-//   K1  each workgroup forms the Gram matrix of its rows in FP64 (products of
-//       FP32 values are exact in FP64); the last workgroup to finish adds the
-//       partials in a fixed order, factors G = R1^T R1 (Cholesky, FP64, one
-//       wave, columns in registers) and stores R1^{-1};
-//   K2  every row is multiplied by R1^{-1} in FP64, Q1 is stored, and the
-//       Gram matrix of Q1 is factored the same way: R2, R = R2 R1;
-//   K3  Q = Q1 R2^{-1}.
-// One pass of Cholesky QR loses orthogonality like kappa^2 u; the second pass
-// starts from kappa(Q1) ~ 1 and restores O(u) for kappa(panel) up to ~1e7
-// (FP64 Gram).  A non-positive pivot (rank deficiency, NaN/Inf) sets status
-// bit 0 and yields a zero column, like the MGS panel.
+// forms, with no sequential loop over columns across the grid.  For an m x w
+// panel (w <= 32), three launches over the same G workgroups (G depends on m
+// only, so every result is bit-reproducible):
+//   K1  each workgroup forms the Gram matrix of its rows with FP64 MFMA
+//       (products of FP32 values are exact in FP64) and stores the packed
+//       upper triangle as its partial;
+//   K2  every workgroup adds the G partials in the same fixed order, factors
+//       G = R1^T R1 and inverts R1 (one elimination, FP64, one wave, columns
+//       in registers), forms q = a R1^{-1} for each of its rows in FP64 and
+//       stores Q1 in FP32, then forms the partial Gram of the stored Q1;
+//   K3  the same for Q1: R2, Q = Q1 R2^{-1}, R = R2 R1.
+// All of the small-matrix arithmetic is FP64, so multiplying by the explicit
+// inverse adds errors of order u_64 kappa, far below the FP32 rounding of Q.
+// Workgroups never wait for each other: the kernel boundaries order the
+// partials, so there are no atomics and no device-scope fences (which would
+// write back and invalidate the L2 of every XCD).
 //
-// The "last workgroup" uses an atomic counter in the caller's workspace header
-// (zeroed when the factorisation starts, reset by the last workgroup), so
-// nothing waits on another workgroup and graph replays see a zero counter.
+// One pass of Cholesky QR loses orthogonality like kappa^2 u; the second pass
+// starts from kappa(Q1) ~ 1 and restores O(u) for kappa(panel) up to ~1e7.
+// A pivot that is not positive and finite, or that cancels to below
+// 64 u_64 of the column's squared norm (numerical rank deficiency in the FP64
+// Gram), sets status bit 0 and yields a zero column, like the MGS panel.
 #include "common.h"
 #include "panel.h"
 
 namespace {
 
-constexpr int kW = 32;          // max panel width
-constexpr int kRows = 256;      // rows per block = threads per workgroup
-constexpr int kMaxGroups = 256; // workgroups (and Gram partials) per launch
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+constexpr int kW = 32;                    // max panel width
+constexpr int kRows = 256;                // rows per block = threads per workgroup
+constexpr int kMaxGroups = 32;            // workgroups (and Gram partials) per launch
+constexpr int kPacked = kW * (kW + 1) / 2;  // packed upper triangle of a 32 x 32 Gram
+constexpr double kPivotRel = 64.0 * 2.220446049250313e-16;
 
 struct CholWork {
-    double *partial;  // kMaxGroups x kW x kW
-    double *rinv;     // kW x kW (current R^{-1}, upper)
-    double *r1;       // kW x kW (first factor)
+    double *p1;  // kMaxGroups x kPacked: partial Grams of A
+    double *p2;  // kMaxGroups x kPacked: partial Grams of Q1
+    double *r1;  // kW x kW, row-major: R1 (upper)
 };
 
 __host__ __device__ inline CholWork carve(float *ws)
 {
     CholWork c;
-    double *d = (double *)ws;
-    c.partial = d;
-    c.rinv = d + (int64_t)kMaxGroups * kW * kW;
-    c.r1 = c.rinv + kW * kW;
+    c.p1 = (double *)ws;
+    c.p2 = c.p1 + (int64_t)kMaxGroups * kPacked;
+    c.r1 = c.p2 + (int64_t)kMaxGroups * kPacked;
     return c;
 }
 
-// Partial Gram of the rows this workgroup owns: G += B^T B over blocks
-// b = g, g + G, ... of the m x w matrix held (per block) in LDS.  Each thread
-// owns a 2 x 2 tile (2bi, 2bj) of G with bi <= bj.
-__device__ __forceinline__ void gram_tile(const double (*Bs)[kW + 1], int rows, int w, double acc[4])
+// Packed index e of the upper triangle (column-major, i <= j) -> (i, j).
+__device__ __forceinline__ void unpack(int e, int &i, int &j)
 {
-    const int t = threadIdx.x;
-    // Tiles of the upper triangle of a 16 x 16 tile grid: t -> (bi, bj).
-    int bi = 0, rem = t;
-    while (bi < 16 && rem >= 16 - bi) { rem -= 16 - bi; ++bi; }
-    if (bi >= 16) return;
-    const int bj = bi + rem;
-    const int i0 = 2 * bi, j0 = 2 * bj;
-    if (i0 >= w || j0 >= w) return;
-    for (int r = 0; r < rows; ++r) {
-        const double a0 = Bs[r][i0], a1 = Bs[r][i0 + 1], b0 = Bs[r][j0], b1 = Bs[r][j0 + 1];
-        acc[0] = fma(a0, b0, acc[0]);
-        acc[1] = fma(a0, b1, acc[1]);
-        acc[2] = fma(a1, b0, acc[2]);
-        acc[3] = fma(a1, b1, acc[3]);
+    int c = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+    while (c * (c + 1) / 2 > e) --c;
+    while ((c + 1) * (c + 2) / 2 <= e) ++c;
+    j = c;
+    i = e - c * (c + 1) / 2;
+}
+
+// The three distinct 16 x 16 tiles of a 32 x 32 Gram: (0,0), (0,1), (1,1).
+struct Gram3 {
+    doublex4 t00, t01, t11;
+};
+
+// v_mfma_f64_16x16x4f64: A (16 x 4) lane l holds A[l%16][l/16], B (4 x 16)
+// lane l holds B[l/16][l%16], D (16 x 16) lane l item r holds
+// D[l/16 + 4r][l%16] (rows interleaved, unlike the FP32 16 x 16 forms).  For the Gram, the k index is a row of the panel: the
+// lane's value of column c = l%16 at row k serves as both A[c][k] and B[k][c].
+// a0 holds columns 0..15, a1 columns 16..31, at four rows (one per MFMA).
+__device__ __forceinline__ void gram_step(Gram3 &g, const double a0[4], const double a1[4])
+{
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        g.t00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[p], a0[p], g.t00, 0, 0, 0);
+        g.t01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[p], a1[p], g.t01, 0, 0, 0);
+        g.t11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[p], a1[p], g.t11, 0, 0, 0);
     }
 }
 
-__device__ __forceinline__ void gram_store(double *P, int w, const double acc[4])
+// Adds the four waves' Grams in wave order and stores this workgroup's
+// packed partial.  Cb: 4 x 3 x 4 x 64 doubles of LDS.
+__device__ __forceinline__ void gram_finish(const Gram3 &g, double *Cb, double *part)
 {
-    const int t = threadIdx.x;
-    int bi = 0, rem = t;
-    while (bi < 16 && rem >= 16 - bi) { rem -= 16 - bi; ++bi; }
-    if (bi >= 16) return;
-    const int bj = bi + rem;
-    const int i0 = 2 * bi, j0 = 2 * bj;
-    const double v[2][2] = {{acc[0], acc[1]}, {acc[2], acc[3]}};
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) {
-            const int i = i0 + a, j = j0 + b;
-            if (i < kW && j < kW) {
-                P[i + j * kW] = v[a][b];
-                P[j + i * kW] = v[a][b];
-            }
-        }
-}
-
-// Last workgroup: sum the partials in a fixed order, Cholesky-factor and
-// invert.  Writes rinv (upper R^{-1}) and, when rout is given, R (upper).
-__device__ void finish_cholesky(const double *partial, int ngroups, int w, double *rinv, double *rout, int *status)
-{
-    __shared__ double G[kW][kW + 1];
-    for (int e = threadIdx.x; e < kW * kW; e += blockDim.x) {
-        double s = 0.0;
-        for (int g = 0; g < ngroups; ++g) s += partial[(int64_t)g * kW * kW + e];
-        G[e % kW][e / kW] = s;
-    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const doublex4 *tl[3] = {&g.t00, &g.t01, &g.t11};
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Cb[((wv * 3 + t) * 4 + r) * 64 + lane] = (*tl[t])[r];
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        // Right-looking Cholesky G = L L^T by one wave.  Lane j keeps rows
-        // k.. of column j of the trailing matrix as a window c[0 .. w-k-1]
-        // (c[t] = G[k + t][j]); the window shifts down one row per step, so
-        // the pivot row is always c[0] and every register index is static.
-        // Lane k writes the finished column k of L to LDS (G[i][k], i >= k).
-        double c[kW];
-#pragma unroll
-        for (int t = 0; t < kW; ++t) c[t] = (lane < w && t < w) ? G[t][lane] : 0.0;
-        bool bad = false;
-        for (int k = 0; k < w; ++k) {
-            const double gkk = __shfl(c[0], k, 64);
-            const bool ok = gkk > 0.0 && gkk < INFINITY;
-            bad |= !ok;
-            const double d = ok ? sqrt(gkk) : 0.0;
-            const double inv = ok ? 1.0 / d : 0.0;
-            const double ljk = c[0] * inv;  // L[j][k] (trailing matrix symmetric)
-#pragma unroll
-            for (int t = 0; t < kW; ++t) {
-                const double lik = __shfl(c[t], k, 64) * inv;  // L[k + t][k]
-                if (lane == k && k + t < w) G[k + t][k] = (t == 0) ? d : lik;
-                if (lane == k && t < k) G[t][k] = 0.0;
-                c[t] = (lane > k && t > 0) ? c[t] - lik * ljk : c[t];
-            }
-#pragma unroll
-            for (int t = 0; t < kW - 1; ++t) c[t] = c[t + 1];
-            c[kW - 1] = 0.0;
-        }
-        if (bad && lane == 0 && status) atomicOr(status, 1);
-        // Columns w..31 of L (unused) are zero.
-        if (lane >= w && lane < kW)
-#pragma unroll
-            for (int i = 0; i < kW; ++i) G[i][lane] = 0.0;
-    }
-    __syncthreads();
-    // G[i][k] = L[i][k] (lower).  R = L^T upper: R[k][j] = L[j][k].
-    // Column j of R^{-1} (upper): solve R x = e_j by back substitution, one
-    // lane per column.
-    if (threadIdx.x < kW) {
-        const int j = threadIdx.x;
-        // x = column j of R^{-1}: R x = e_j, R[i][k] = L[k][i]; x[i] = 0 for
-        // i > j, so the static loops below never read an unset entry.
-        double x[kW];
-#pragma unroll
-        for (int i = kW - 1; i >= 0; --i) {
-            double s = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-            for (int k = i + 1; k < kW; ++k) s -= G[k][i] * x[k];
-            const double rii = G[i][i];
-            x[i] = (i <= j && j < w && rii != 0.0) ? s / rii : 0.0;
-        }
-#pragma unroll
-        for (int i = 0; i < kW; ++i) rinv[i + j * kW] = x[i];
-        if (rout)
-#pragma unroll
-            for (int i = 0; i < kW; ++i) rout[i + j * kW] = (i <= j && j < w) ? G[j][i] : 0.0;
+    for (int e = threadIdx.x; e < kPacked; e += kRows) {
+        int i, j;
+        unpack(e, i, j);
+        const int t = (i >> 4) + (j >> 4);  // (0,0) -> 0, (0,1) -> 1, (1,1) -> 2
+        const int ii = i & 15, jj = j & 15;
+        const int ln = jj + 16 * (ii & 3), r = ii >> 2;
+        double s = Cb[((0 * 3 + t) * 4 + r) * 64 + ln];
+        s += Cb[((1 * 3 + t) * 4 + r) * 64 + ln];
+        s += Cb[((2 * 3 + t) * 4 + r) * 64 + ln];
+        s += Cb[((3 * 3 + t) * 4 + r) * 64 + ln];
+        part[e] = s;
     }
 }
 
-// Loads block b of the m x w panel (rows b*256 ..) into LDS as FP64.
-__device__ __forceinline__ int load_block(const float *A, int64_t lda, int m, int w, int b, double (*Bs)[kW + 1])
+// G (LDS, 32 x 33) = sum over g < ngroups of the packed partials, in order
+// g = 0, 1, ...; entries outside the w x w block are the identity's.
+__device__ __forceinline__ void sum_partials(const double *part, int ngroups, int w, double (*Gs)[kW + 1])
 {
-    const int r0 = b * kRows;
-    const int rows = min(kRows, m - r0);
+    constexpr int kPer = (kPacked + kRows - 1) / kRows;  // 3
+    int ei[kPer];
+    double s[kPer];
+#pragma unroll
+    for (int x = 0; x < kPer; ++x) {
+        ei[x] = min((int)threadIdx.x + x * kRows, kPacked - 1);
+        s[x] = 0.0;
+    }
+    // Up to 48 loads in flight per thread: the partials were written by other
+    // XCDs and come from memory, so each round costs a full memory latency.
+    constexpr int kBatch = 16;
+    for (int g0 = 0; g0 < ngroups; g0 += kBatch) {
+        double v[kPer][kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int g = min(g0 + u, ngroups - 1);
+#pragma unroll
+            for (int x = 0; x < kPer; ++x) v[x][u] = part[(int64_t)g * kPacked + ei[x]];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+#pragma unroll
+            for (int x = 0; x < kPer; ++x)
+                if (g0 + u < ngroups) s[x] += v[x][u];
+    }
+#pragma unroll
+    for (int x = 0; x < kPer; ++x) {
+        const int e = threadIdx.x + x * kRows;
+        if (e < kPacked) {
+            int i, j;
+            unpack(e, i, j);
+            double val = s[x];
+            if (i >= w || j >= w) val = (i == j) ? 1.0 : 0.0;
+            Gs[i][j] = val;
+            Gs[j][i] = val;
+        }
+    }
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Cholesky G = R^T R of the 32 x 32 Gram in LDS by wave 0, and R^{-1}, in
+// one right-looking elimination, fully unrolled.  Lane j < 32 keeps column j
+// of the trailing matrix in registers; lane 32 + c keeps column c of the
+// identity, to which the same row operations are applied (Gauss-Jordan on
+// [G | I]): it ends as column c of the unit-lower M with M G = D L^T, and
+// R^{-1}[c][k] = M[k][c] / sqrt(d_k).  Column k's multipliers G[i][k] are read
+// from lane k with v_readlane and serve both halves.
+// Writes Rs[k][j] = R[k][j] (upper) and Ris[c][k] = R^{-1}[c][k] (upper).  A
+// bad pivot zeroes row and column k of both.  Returns true if a pivot was bad.
+__device__ __forceinline__ bool cholesky_wave(const double (*Gs)[kW + 1], double (*Rs)[kW], double (*Ris)[kW])
+{
+    const int lane = threadIdx.x;
+    const bool aug = lane >= kW;
+    const int j = lane & (kW - 1);
+    double g[kW];
+#pragma unroll
+    for (int i = 0; i < kW; ++i) g[i] = aug ? (i == j ? 1.0 : 0.0) : Gs[i][j];
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+        const double p = readlane_f64(g[k], k);
+        const double p0 = Gs[k][k];
+        const bool ok = p0 > 0.0 && p > kPivotRel * p0 && p <= 1.7976931348623157e308;
+        bad |= !ok;
+        // 1/sqrt(p): hardware estimate and two Newton steps.
+        const double pp = ok ? p : 1.0;
+        double y = __builtin_amdgcn_rsq(pp);
+        y = y * fma(-0.5 * pp * y, y, 1.5);
+        y = y * fma(-0.5 * pp * y, y, 1.5);
+        const double inv = ok ? y : 0.0;
+        const double d = ok ? p * y : 0.0;
+        const double val = g[k] * inv;
+        if (aug)
+            Ris[j][k] = val;
+        else
+            Rs[k][j] = (j < k) ? 0.0 : (j == k ? d : val);
+        const double a = val * inv;  // row k of the trailing matrix (or of M) / p
+#pragma unroll
+        for (int i = k + 1; i < kW; ++i) g[i] = fma(-readlane_f64(g[i], k), a, g[i]);
+    }
+    return bad;
+}
+
+// Loads row r (clamped) of the m x w panel as 32 floats (zero beyond w, m).
+__device__ __forceinline__ void load_row(const float *A, int64_t lda, int m, int w, int row, float v[kW])
+{
+    const int rc = min(row, m - 1);
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+        const float x = A[rc + (int64_t)min(k, w - 1) * lda];
+        v[k] = (row < m && k < w) ? x : 0.0f;
+    }
+}
+
+// q = a R^{-1} for one row in FP64 (R^{-1} upper, in LDS).  Row c + 1 of
+// R^{-1} is read while row c is applied.  The scheduling barrier and the
+// empty asm statements keep every step's reads and updates in that step:
+// hoisted or sunk, all of R^{-1} would be live at once and would not fit in
+// registers.
+__device__ __forceinline__ void apply_row(const float v[kW], const double (*Ris)[kW], double q[kW])
+{
+    double cur[kW], nxt[kW];
+#pragma unroll
+    for (int jj = 0; jj < kW; ++jj) {
+        q[jj] = 0.0;
+        cur[jj] = Ris[0][jj];
+    }
+#pragma unroll
+    for (int c = 0; c < kW; ++c) {
+        __builtin_amdgcn_sched_barrier(0);
+        if (c + 1 < kW)
+#pragma unroll
+            for (int jj = c + 1; jj < kW; ++jj) nxt[jj] = Ris[c + 1][jj];
+        const double x = (double)v[c];
+#pragma unroll
+        for (int jj = c; jj < kW; ++jj) q[jj] = fma(x, cur[jj], q[jj]);
+#pragma unroll
+        for (int jj = c; jj < kW; ++jj) asm volatile("" : "+v"(q[jj]));
+#pragma unroll
+        for (int jj = c + 1; jj < kW; ++jj) cur[jj] = nxt[jj];
+    }
+}
+
+// Gram MFMA operands of one 16-row group at local row r0 + 4 (l/16) of a
+// column-major LDS block Qs[col][kRows + 4].
+__device__ __forceinline__ void gram_lds(Gram3 &g, const float (*Qs)[kRows + 4], int r0)
+{
+    const int lane = threadIdx.x & 63;
+    const int c = lane & 15, r = r0 + 4 * (lane >> 4);
+    const floatx4 x0 = *(const floatx4 *)&Qs[c][r];
+    const floatx4 x1 = *(const floatx4 *)&Qs[16 + c][r];
+    double a0[4], a1[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        a0[p] = (double)x0[p];
+        a1[p] = (double)x1[p];
+    }
+    gram_step(g, a0, a1);
+}
+
+__global__ __launch_bounds__(kRows) void chol_gram_kernel(int m, int w, const float *A, int64_t lda, float *ws)
+{
+    __shared__ double Cb[4 * 3 * 4 * 64];
+    const CholWork cw = carve(ws);
+    const int nblk = (m + kRows - 1) / kRows;
+    const int G = gridDim.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = lane & 15;
+    const int64_t col0 = (int64_t)min(c, w - 1) * lda, col1 = (int64_t)min(16 + c, w - 1) * lda;
+    const bool ok0 = c < w, ok1 = 16 + c < w;
+    Gram3 g;
+    g.t00 = g.t01 = g.t11 = doublex4{0.0, 0.0, 0.0, 0.0};
+    for (int b = blockIdx.x; b < nblk; b += G) {
+#pragma unroll
+        for (int sg = 0; sg < 4; ++sg) {
+            const int row0 = b * kRows + wv * 64 + sg * 16 + 4 * (lane >> 4);
+            double a0[4], a1[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const int row = row0 + p, rc = min(row, m - 1);
+                const float x0 = A[rc + col0], x1 = A[rc + col1];
+                a0[p] = (row < m && ok0) ? (double)x0 : 0.0;
+                a1[p] = (row < m && ok1) ? (double)x1 : 0.0;
+            }
+            gram_step(g, a0, a1);
+        }
+    }
+    gram_finish(g, Cb, cw.p1 + (int64_t)blockIdx.x * kPacked);
+}
+
+// One Cholesky QR pass over the rows of this workgroup.
+//   PASS 1: partials p1 (of A) -> R1 (stored to r1 by workgroup 0); Q1 = A R1^{-1}
+//           stored; partials p2 of the stored Q1.
+//   PASS 2: partials p2 -> R2; Q = Q1 R2^{-1} stored; workgroup 0 writes
+//           R = R2 R1 (ldr, FP32).
+template <int PASS>
+__global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *A, int64_t lda, float *ws, float *R,
+                                                          int64_t ldr, int *status)
+{
+    __shared__ double Gs[kW][kW + 1];
+    __shared__ double Rs[kW][kW];
+    __shared__ double Ris[kW][kW];
+    __shared__ float Qs[PASS == 1 ? kW : 1][kRows + 4];
+    __shared__ double Cb[PASS == 1 ? 4 * 3 * 4 * 64 : 1];
+    const CholWork cw = carve(ws);
+    const int nblk = (m + kRows - 1) / kRows;
+    const int G = gridDim.x;
     const int t = threadIdx.x;
-    const int rc = min(t, rows - 1);
+
+    // The first block's rows are in flight while the factor is formed.
     float v[kW];
-#pragma unroll
-    for (int k = 0; k < kW; ++k) v[k] = A[r0 + rc + (int64_t)min(k, w - 1) * lda];
-#pragma unroll
-    for (int k = 0; k < kW; ++k) Bs[t][k] = (t < rows && k < w) ? (double)v[k] : 0.0;
-    return rows;
-}
+    load_row(A, lda, m, w, blockIdx.x * kRows + t, v);
 
-// Signals completion of this workgroup's partial; true for the last one.
-__device__ __forceinline__ bool arrive_last(int *counter, int ngroups)
-{
-    __shared__ int last;
-    __threadfence();
+    sum_partials(PASS == 1 ? cw.p1 : cw.p2, G, w, Gs);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const int old = atomicAdd(counter, 1);
-        last = (old == ngroups - 1);
-        if (last) *counter = 0;  // reset for the next panel (and graph replays)
+    if (t < 64) {
+        const bool bad = cholesky_wave(Gs, Rs, Ris);
+        if (bad && t == 0 && blockIdx.x == 0 && status) atomicOr(status, 1);
     }
     __syncthreads();
-    if (last) __threadfence();
-    return last;
-}
-
-__global__ __launch_bounds__(kRows) void chol_gram_kernel(int m, int w, const float *A, int64_t lda, float *ws,
-                                                         int *counter, int *status)
-{
-    __shared__ double Bs[kRows][kW + 1];
-    const CholWork cw = carve(ws);
-    const int nblk = (m + kRows - 1) / kRows;
-    const int G = gridDim.x;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = blockIdx.x; b < nblk; b += G) {
-        const int rows = load_block(A, lda, m, w, b, Bs);
-        __syncthreads();
-        gram_tile(Bs, rows, w, acc);
-        __syncthreads();
-    }
-    gram_store(cw.partial + (int64_t)blockIdx.x * kW * kW, w, acc);
-    if (arrive_last(counter, G)) finish_cholesky(cw.partial, G, w, cw.rinv, cw.r1, status);
-}
-
-// Q1 = A R1^{-1} (stored), then the Gram of Q1 (from the FP64 values) is
-// factored: R2; R (ldr) = R2 R1 in FP32.  With gram = false only Q1 is formed.
-template <bool GRAM>
-__global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *A, int64_t lda, float *ws,
-                                                          float *R, int64_t ldr, int *counter, int *status)
-{
-    __shared__ double Bs[GRAM ? kRows : 1][kW + 1];
-    __shared__ double Ri[kW][kW + 1];
-    const CholWork cw = carve(ws);
-    for (int e = threadIdx.x; e < kW * kW; e += kRows) Ri[e % kW][e / kW] = cw.rinv[e];
-    __syncthreads();
-    const int nblk = (m + kRows - 1) / kRows;
-    const int G = gridDim.x;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = blockIdx.x; b < nblk; b += G) {
-        const int r0 = b * kRows;
-        const int rows = min(kRows, m - r0);
-        const int t = threadIdx.x;
-        const int rc = min(t, rows - 1);
-        float v[kW];
-#pragma unroll
-        for (int k = 0; k < kW; ++k) v[k] = A[r0 + rc + (int64_t)min(k, w - 1) * lda];
-        double q[kW];
-#pragma unroll
-        for (int j = 0; j < kW; ++j) q[j] = 0.0;
-#pragma unroll
-        for (int k = 0; k < kW; ++k) {
-            if (k >= w) break;
-            const double vk = (double)v[k];
-#pragma unroll
-            for (int j = 0; j < kW; ++j)
-                if (j >= k) q[j] = fma(vk, Ri[k][j], q[j]);
-        }
-        if (t < rows)
-#pragma unroll
-            for (int j = 0; j < kW; ++j)
-                if (j < w) A[r0 + t + (int64_t)j * lda] = (float)q[j];
-        if constexpr (GRAM) {
-#pragma unroll
-            for (int j = 0; j < kW; ++j) Bs[t][j] = (t < rows && j < w) ? q[j] : 0.0;
+    if (blockIdx.x == 0) {
+        if constexpr (PASS == 1) {
+            for (int e = t; e < kW * kW; e += kRows) cw.r1[e] = Rs[e / kW][e % kW];
+        } else {
+            // R = R2 R1 (both upper), FP64 then rounded once; R1 staged in
+            // LDS (over the Gram, no longer needed).
+            for (int e = t; e < kW * kW; e += kRows) Gs[e / kW][e % kW] = cw.r1[e];
             __syncthreads();
-            gram_tile(Bs, rows, w, acc);
-            __syncthreads();
-        }
-    }
-    if constexpr (GRAM) {
-        gram_store(cw.partial + (int64_t)blockIdx.x * kW * kW, w, acc);
-        if (arrive_last(counter, G)) {
-            __shared__ double R2[kW * kW];
-            finish_cholesky(cw.partial, G, w, cw.rinv, R2, status);
-            __syncthreads();
-            // R = R2 R1 (both upper), FP64 then rounded once.
-            for (int e = threadIdx.x; e < w * w; e += kRows) {
+            for (int e = t; e < w * w; e += kRows) {
                 const int i = e % w, j = e / w;
                 double s = 0.0;
-                for (int k = i; k <= j; ++k) s += R2[i + k * kW] * cw.r1[k + j * kW];
+                for (int k = i; k <= j; ++k) s = fma(Rs[i][k], Gs[k][j], s);
                 R[i + (int64_t)j * ldr] = (i <= j) ? (float)s : 0.0f;
             }
         }
     }
+
+    Gram3 g;
+    g.t00 = g.t01 = g.t11 = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int b = blockIdx.x; b < nblk; b += G) {
+        // Also keeps the loop-invariant reads of R inside the loop: hoisted,
+        // 528 doubles would not fit in registers.
+        __syncthreads();
+        const int r0 = b * kRows;
+        const int rows = min(kRows, m - r0);
+        double q[kW];
+        apply_row(v, Ris, q);
+        float qf[kW];
+#pragma unroll
+        for (int jj = 0; jj < kW; ++jj) qf[jj] = (t < rows && jj < w) ? (float)q[jj] : 0.0f;
+        if (t < rows)
+#pragma unroll
+            for (int jj = 0; jj < kW; ++jj)
+                if (jj < w) A[r0 + t + (int64_t)jj * lda] = qf[jj];
+        if (b + G < nblk) load_row(A, lda, m, w, (b + G) * kRows + t, v);
+        if constexpr (PASS == 1) {
+            // Gram of the stored (FP32) Q1 rows of this block.
+#pragma unroll
+            for (int jj = 0; jj < kW; ++jj) Qs[jj][t] = qf[jj];
+            __syncthreads();
+            const int wv = t >> 6;
+#pragma unroll
+            for (int sg = 0; sg < 4; ++sg) gram_lds(g, Qs, wv * 64 + sg * 16);
+            __syncthreads();
+        }
+    }
+    if constexpr (PASS == 1) gram_finish(g, Cb, cw.p2 + (int64_t)blockIdx.x * kPacked);
 }
 
 } // namespace
 
-int64_t panel_chol_work_floats() { return ((int64_t)kMaxGroups * kW * kW + 2 * kW * kW) * 2 + 64; }
+int64_t panel_chol_work_floats() { return ((int64_t)2 * kMaxGroups * kPacked + kW * kW) * 2 + 64; }
 
 int panel_cholqr2(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr, float *work, int64_t lwork,
-                  int *counters, int *status, int *launches)
+                  int *status, int *launches)
 {
     if (w < 1 || w > kW || m < w) return LATER_ERR_SHAPE;
     if (lwork < panel_chol_work_floats()) return LATER_ERR_WORKSPACE;
     const int nblk = (m + kRows - 1) / kRows;
     const int G = nblk < kMaxGroups ? nblk : kMaxGroups;
-    hipLaunchKernelGGL(chol_gram_kernel, dim3(G), dim3(kRows), 0, s, m, w, (const float *)A, (int64_t)lda, work,
-                       counters, status);
-    hipLaunchKernelGGL(chol_apply_kernel<true>, dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
-                       (int64_t)ldr, counters, status);
-    hipLaunchKernelGGL(chol_apply_kernel<false>, dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
-                       (int64_t)ldr, counters, status);
+    hipLaunchKernelGGL(chol_gram_kernel, dim3(G), dim3(kRows), 0, s, m, w, (const float *)A, (int64_t)lda, work);
+    hipLaunchKernelGGL(chol_apply_kernel<1>, dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
+                       (int64_t)ldr, status);
+    hipLaunchKernelGGL(chol_apply_kernel<2>, dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
+                       (int64_t)ldr, status);
     HIP_OK(hipGetLastError());
     if (launches) *launches += 3;
     return LATER_OK;

@@ -124,8 +124,7 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     if (depth > r.depth) r.depth = depth;
     if (n <= 32) {
         if (r.o.panel == LATER_PANEL_CHOLQR2)
-            return panel_cholqr2(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n, r.status + 1, r.status,
-                                 &r.launches);
+            return panel_cholqr2(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n, r.status, &r.launches);
         return panel_tsqr(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n,
                           r.o.panel == LATER_PANEL_MGS64, r.status, &r.launches);
     }
@@ -344,8 +343,8 @@ LATER_API int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, 
     hipStream_t s = ctx_stream(ctxt);
     hipLaunchKernelGGL(status_clear_kernel, dim3(1), dim3(64), 0, s, (int *)work);
     if (fp64_sums == 2)
-        return panel_cholqr2(s, m, w, A, lda, R, ldr, work + kHeaderFloats, need - kHeaderFloats, (int *)work + 1,
-                             (int *)work, nullptr);
+        return panel_cholqr2(s, m, w, A, lda, R, ldr, work + kHeaderFloats, need - kHeaderFloats, (int *)work,
+                             nullptr);
     return panel_tsqr(s, m, w, A, lda, R, ldr, work + kHeaderFloats, need - kHeaderFloats, fp64_sums != 0,
                       (int *)work, nullptr);
 }
