@@ -95,7 +95,9 @@ typedef struct later_opts {
 /* Defaults reproduce the reference form: FP16 GEMMs above 128 columns, FP32
  * below, FP32 block-MGS TSQR panel, no re-orthogonalisation. */
 void later_opts_default(later_opts *o);
-/* The accuracy-recovering mixed-precision form. */
+/* The accuracy-recovering mixed-precision form: FP16X3 first pass (FP64
+ * folds every 256 rows), FP16X6 second-pass projections (folds every 32
+ * rows), Cholesky QR twice at the leaves, second full pass. */
 void later_opts_mixed(later_opts *o);
 /* The plain-FP32 form: FP32 MFMA everywhere, reference panel, one pass. */
 void later_opts_fp32(later_opts *o);
@@ -143,7 +145,9 @@ int later_plan_destroy(void *plan);
  * grouped launch per step of the shared recursion tree, one stream per
  * bucket) and captures them into a HIP graph -- it allocates its tables with
  * hipMalloc once; later_batch_plan_execute replays the graph on `stream`.
- * Options as for later_rgsqrf_ex except LATER_REORTH_NODE.  Matrix i's
+ * Options as for later_rgsqrf_ex except LATER_REORTH_NODE; the leaves always
+ * run the block-MGS tree, with FP64 sums for LATER_PANEL_MGS64 and
+ * LATER_PANEL_CHOLQR2.  Matrix i's
  * status word (bit 0: breakdown) is the first int of its workspace block. */
 int later_batch_workspace(int count, const int *m, const int *n, const later_opts *opts, int64_t *lwork);
 int later_batch_plan_create(int count, const int *m, const int *n, float *const *A, const int *lda,

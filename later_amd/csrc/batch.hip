@@ -133,9 +133,11 @@ struct Builder {
     }
 
     // One TSQR level across matrices: MGS (or apply) of the listed panels.
+    // Batched leaves always use the block-MGS tree; the accurate panels
+    // (LATER_PANEL_MGS64 and LATER_PANEL_CHOLQR2) run it with FP64 sums.
     void panel(bool apply, const std::vector<PanelArgs> &a)
     {
-        Launch L{apply ? L_APPLY : L_MGS, 0, o.panel == LATER_PANEL_MGS64, (int)ptasks.size(), 0};
+        Launch L{apply ? L_APPLY : L_MGS, 0, o.panel != LATER_PANEL_MGS, (int)ptasks.size(), 0};
         for (const PanelArgs &p : a) {
             const int ai = (int)pargs.size();
             pargs.push_back(p);
@@ -215,7 +217,7 @@ struct Builder {
             g.C = Rt + c0 + (int64_t)(c0 + n1) * ldr; g.ldc = ldr;
             g.a_exp = aexp; g.b_exp = sc ? M->exps : nullptr;
             g.kper = (M->m + 31) / 32 * 32;
-            g.chunk_tiles = ch64 ? chunk / 32 : 0;
+            g.chunk_rows = ch64 ? chunk : 0;
             g.partial = nullptr; g.update = 0; g.alpha = 1.0f; g.tri = 0;
             pa.push_back(g);
         }
@@ -243,7 +245,7 @@ struct Builder {
             g.C = (float *)p.B; g.ldc = M->lda;
             g.a_exp = aexp; g.b_exp = p.b_exp;
             g.kper = (n1 + 31) / 32 * 32;
-            g.chunk_tiles = ch64 ? (n1 + 31) / 32 : 0;
+            g.chunk_rows = ch64 ? kFoldAtEnd : 0;
             g.partial = nullptr; g.update = 1; g.alpha = -1.0f; g.tri = 0;
             ua.push_back(g);
         }
@@ -300,7 +302,7 @@ struct Builder {
                 g.A = M->R2; g.lda = M->n; g.B = M->T; g.ldb = M->n; g.C = M->R; g.ldc = M->ldr;
                 g.a_exp = scaled(p2) ? kQExp : 0; g.b_exp = scaled(p2) ? M->exps : nullptr;
                 g.kper = (M->n + 31) / 32 * 32;
-                g.chunk_tiles = o.chunk > 0 ? (M->n + 31) / 32 : 0;
+                g.chunk_rows = o.chunk > 0 ? kFoldAtEnd : 0;
                 g.partial = nullptr; g.update = 1; g.alpha = 1.0f; g.tri = 1;
                 ta.push_back(g);
             }

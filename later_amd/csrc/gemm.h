@@ -14,13 +14,16 @@ struct GemmArgs {
     float *C; int64_t ldc;         // M x N
     int a_exp;                     // op(A) scaled by 2^a_exp before the cast
     const int *b_exp;              // column j of B scaled by 2^b_exp[j] (nullptr: none)
-    int kper;                      // K rows per split (multiple of BK)
-    int chunk_tiles;               // BK tiles per FP64 fold (0: no FP64 accumulation)
+    int kper;                      // K rows per split (multiple of 64)
+    int chunk_rows;                // K rows per FP64 fold, a multiple of 32 (0: no FP64 accumulation)
     void *partial;                 // split-K partials (nsplit x M x N), nullptr: write C
     int update;                    // 0: C = result; 1: C += alpha * result
     float alpha;
     int tri;                       // op(A), B upper triangular: skip zero tiles and K ranges
 };
+
+// chunk_rows that folds into FP64 once, after the last K row.
+constexpr int kFoldAtEnd = 1 << 30;
 
 struct ColexpArgs {
     int m;

@@ -19,8 +19,19 @@
 
 namespace {
 
-constexpr int BK = 32;
 constexpr int NT = 256;  // 4 waves in a 2 x 2 grid
+constexpr int kMaxBK = 64;
+
+// K depth of one LDS slab.  64 for the split-FP16 forms without the FP64
+// fold (their three or six products per slab then cover the next slab's
+// global-load latency; measured +20% on the node updates); 32 elsewhere: for
+// FP16/BF16 the deeper slab halves the workgroups per CU, for FP32 its LDS
+// tiles are twice as big, and with the FP64 fold the extra staging
+// registers spill.
+template <int PREC, bool CH64> constexpr int bk()
+{
+    return ((PREC == GEMM_F16X3 || PREC == GEMM_F16X6) && !CH64) ? 64 : 32;
+}
 
 template <int PREC> struct Lds;
 template <> struct Lds<GEMM_F32> { typedef float T; };
@@ -40,7 +51,7 @@ template <int PREC> constexpr int pieces() { return PREC == GEMM_F16X6 ? 3 : PRE
 // Loads of one BK-deep slab of an R-row operand tile into registers.
 //   KM (rows contiguous along K): thread owns SEG consecutive k of one row.
 //   !KM (rows contiguous along M): thread owns 4 consecutive rows x KG k.
-template <int R, bool KM> struct Slab {
+template <int R, bool KM, int BK> struct Slab {
     static constexpr int NV = R * BK / NT;
     static constexpr int SEG = NV;              // KM
     static constexpr int TPR = BK / SEG;        // KM: threads per row
@@ -112,7 +123,7 @@ template <int R, bool KM> struct Slab {
 
 // LDS tile layouts.  16-bit types: [row][BK+8] (k contiguous, 16-byte MFMA
 // fragment reads).  FP32: [k][R+4] (row contiguous, 4-byte fragment reads).
-template <int PREC, int R> struct Tile {
+template <int PREC, int R, int BK> struct Tile {
     typedef typename Lds<PREC>::T T;
     static constexpr bool F32 = (PREC == GEMM_F32);
     static constexpr int LD = F32 ? (R + 4) : (BK + 8);
@@ -131,11 +142,11 @@ template <typename T, int N> struct Vec {
 // a K-major slab (SEG consecutive k of one row) goes out as 16-byte vectors
 // and an M-major slab (KG consecutive k for each of 4 rows) as KG-wide ones;
 // FP32 keeps rows contiguous per k, so an M-major slab writes 4-row vectors.
-template <int PREC, int R, bool KM>
-__device__ __forceinline__ void store_slab(Tile<PREC, R> &t, const Slab<R, KM> &s, float scale)
+template <int PREC, int R, bool KM, int BK>
+__device__ __forceinline__ void store_slab(Tile<PREC, R, BK> &t, const Slab<R, KM, BK> &s, float scale)
 {
     typedef typename Lds<PREC>::T T;
-    typedef Slab<R, KM> S;
+    typedef Slab<R, KM, BK> S;
     if constexpr (PREC == GEMM_F32) {
         if constexpr (KM) {
 #pragma unroll
@@ -200,11 +211,11 @@ __device__ __forceinline__ floatx16 mfma(typename Frag<PREC>::T a, typename Frag
 // Fragment of 32 rows starting at `row0` for k-step `ks`.
 // 32x32x16 (16-bit): lane l supplies row l%32, k = 16 ks + 8 (l/32) + 0..7.
 // 32x32x2 (FP32):    lane l supplies row l%32, k = 2 ks + l/32.
-template <int PREC, int R>
+template <int PREC, int R, int BK>
 __device__ __forceinline__ typename Frag<PREC>::T frag(const typename Lds<PREC>::T *base, int row0, int ks)
 {
     const int lane = threadIdx.x & 63;
-    typedef Tile<PREC, R> TT;
+    typedef Tile<PREC, R, BK> TT;
     if constexpr (PREC == GEMM_F32) {
         return base[(2 * ks + (lane >> 5)) * TT::LD + row0 + (lane & 31)];
     } else {
@@ -219,11 +230,13 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
     constexpr int FM = WM / 32, FN = WN / 32;   // 32x32 MFMA blocks per wave
     constexpr int NP = pieces<PREC>();          // FP16 pieces per value
     constexpr int NG = NP;                       // accumulator groups
-    constexpr int KSTEPS = (PREC == GEMM_F32) ? BK / 2 : BK / 16;
+    constexpr int BK = bk<PREC, CH64>();
+    constexpr int KR = (PREC == GEMM_F32) ? 2 : 16;  // K rows per MFMA step
+    constexpr int KSTEPS = BK / KR;
 
     // Double-buffered operand tiles: one barrier per BK slab.
-    __shared__ Tile<PREC, BM> As[2];
-    __shared__ Tile<PREC, BN> Bs[2];
+    __shared__ Tile<PREC, BM, BK> As[2];
+    __shared__ Tile<PREC, BN, BK> Bs[2];
 
     const int m0 = tm * BM, n0 = tn * BN;
     int kbeg = split * g.kper;
@@ -243,8 +256,8 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
 
     // Operand scales (powers of two, so exact).  A has one global scale; a
     // thread's B values all lie in one column of B (B rows are K-contiguous).
-    Slab<BM, AKM> sa;
-    Slab<BN, true> sb;
+    Slab<BM, AKM, BK> sa;
+    Slab<BN, true, BK> sb;
     const float ascale = ldexpf(1.0f, g.a_exp);
     const int bcol = n0 + sb.row_of(0);
     const float bscale = (g.b_exp && bcol < g.N) ? ldexpf(1.0f, g.b_exp[bcol]) : 1.0f;
@@ -297,8 +310,8 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
     if (ntiles > 0) {
         sa.load(g.A, g.lda, m0, g.M, kbeg, kend, avec);
         sb.load(g.B, g.ldb, n0, g.N, kbeg, kend, bvec);
-        store_slab<PREC, BM, AKM>(As[0], sa, ascale);
-        store_slab<PREC, BN, true>(Bs[0], sb, bscale);
+        store_slab<PREC, BM, AKM, BK>(As[0], sa, ascale);
+        store_slab<PREC, BN, true, BK>(Bs[0], sb, bscale);
         __syncthreads();
     }
     for (int it = 0; it < ntiles; ++it) {
@@ -315,9 +328,9 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
 #pragma unroll
             for (int c = 0; c < NP; ++c) {
 #pragma unroll
-                for (int i = 0; i < FM; ++i) fa[c][i] = frag<PREC, BM>(As[cur].p[c], wm * WM + 32 * i, ks);
+                for (int i = 0; i < FM; ++i) fa[c][i] = frag<PREC, BM, BK>(As[cur].p[c], wm * WM + 32 * i, ks);
 #pragma unroll
-                for (int j = 0; j < FN; ++j) fb[c][j] = frag<PREC, BN>(Bs[cur].p[c], wn * WN + 32 * j, ks);
+                for (int j = 0; j < FN; ++j) fb[c][j] = frag<PREC, BN, BK>(Bs[cur].p[c], wn * WN + 32 * j, ks);
             }
             // Group g gets products p_a q_b with a + b = g; the loops run over
             // the blocks innermost so that updates of one accumulator are
@@ -331,11 +344,13 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
 #pragma unroll
                         for (int j = 0; j < FN; ++j)
                             acc[gq][i][j] = mfma<PREC>(fb[gq - a][j], fa[a][i], acc[gq][i][j]);
+            // FP64 fold every chunk_rows rows of K (a multiple of 32).
+            if (CH64 && ks + 1 < KSTEPS && (it * BK + (ks + 1) * KR) % g.chunk_rows == 0) fold();
         }
-        if (CH64 && ((it + 1) % g.chunk_tiles == 0 || it + 1 == ntiles)) fold();
+        if (CH64 && (((it + 1) * BK) % g.chunk_rows == 0 || it + 1 == ntiles)) fold();
         if (more) {
-            store_slab<PREC, BM, AKM>(As[cur ^ 1], sa, ascale);
-            store_slab<PREC, BN, true>(Bs[cur ^ 1], sb, bscale);
+            store_slab<PREC, BM, AKM, BK>(As[cur ^ 1], sa, ascale);
+            store_slab<PREC, BN, true, BK>(Bs[cur ^ 1], sb, bscale);
         }
         __syncthreads();
     }
@@ -374,10 +389,27 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
     }
 }
 
+// Tile order.  Workgroups are dispatched round-robin over the 8 XCDs, each
+// with its own L2: the linear id is renumbered so that every XCD works
+// through a contiguous range, and each range walks the tiles in groups of 8
+// tile-rows (column by column inside a group), so the workgroups resident on
+// one XCD share their A and B tiles in its L2.
 template <int PREC, int BM, int BN, bool AKM, bool CH64>
 __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
 {
-    gemm_body<PREC, BM, BN, AKM, CH64>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+    constexpr int kXcd = 8, kGroup = 8;
+    const int nx = gridDim.x, ny = gridDim.y;
+    const int total = nx * ny * gridDim.z;
+    int L = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+    const int q = total / kXcd;
+    if (L < kXcd * q) L = (L % kXcd) * q + L / kXcd;
+    const int tiles = nx * ny;
+    const int split = L / tiles;
+    const int r = L % tiles;
+    const int first = (r / (kGroup * ny)) * kGroup;
+    const int gsz = min(nx - first, kGroup);
+    const int rr = r - first * ny;
+    gemm_body<PREC, BM, BN, AKM, CH64>(g, first + rr % gsz, rr / gsz, split);
 }
 
 // Grouped launch: workgroup b computes tile (t.y, t.z) of split t.w of the
@@ -543,7 +575,7 @@ static void proj_plan(int m, int n1, int n2, int chunk, bool *small, int *nsplit
     if (ns > budget) ns = budget;
     if (ns < 1) ns = 1;
     int64_t kp = ceil_div64(m, ns);
-    const int gran = (chunk > 0) ? chunk : BK;
+    const int gran = (chunk > kMaxBK) ? chunk : kMaxBK;
     kp = ceil_div64(kp, gran) * gran;
     *nsplit = (int)ceil_div64(m, kp);
     *kper = (int)kp;
@@ -562,7 +594,7 @@ int gemm_proj(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, in
               int64_t wfloats, int *launches)
 {
     const bool ch64 = chunk > 0;
-    if (ch64 && chunk % BK) return LATER_ERR_OPTION;
+    if (ch64 && (chunk % 32 || (chunk < kMaxBK && kMaxBK % chunk))) return LATER_ERR_OPTION;
     bool small;
     int ns, kp;
     proj_plan(m, n1, n2, chunk, &small, &ns, &kp);
@@ -572,7 +604,7 @@ int gemm_proj(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, in
     g.A = Q; g.lda = ldq; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
     g.a_exp = a_exp; g.b_exp = b_exp;
     g.kper = kp;
-    g.chunk_tiles = ch64 ? chunk / BK : 0;
+    g.chunk_rows = ch64 ? chunk : 0;
     g.partial = (ns > 1) ? (void *)work : nullptr;
     g.update = 0;
     g.alpha = 1.0f;
@@ -608,7 +640,7 @@ int gemm_nn(hipStream_t s, int prec, int M, int N, int K, float alpha, const flo
     g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
     g.a_exp = a_exp; g.b_exp = b_exp;
     g.kper = K;
-    g.chunk_tiles = f64 ? (int)ceil_div64(K, BK) : 0;
+    g.chunk_rows = f64 ? kFoldAtEnd : 0;
     g.partial = nullptr;
     g.update = 1;
     g.alpha = alpha;
