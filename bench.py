@@ -46,6 +46,7 @@ def time_form(later, torch, A0, form, steps, warmup, graph=True):
     m, n = A0.shape
     o = later.opts(form)
     work = later.workspace(m, n, o)
+    hwork = later.hworkspace(m, n, o)  # 16-bit operand staging (None for the FP32 form)
     A = later.colmajor(m, n)
     R = later.colmajor(n, n)
     s = torch.cuda.current_stream()
@@ -60,11 +61,11 @@ def time_form(later, torch, A0, form, steps, warmup, graph=True):
         torch.cuda.synchronize()
         return ev0.elapsed_time(ev1)
 
-    eager_ms = one(lambda: later.rgsqrf(A, R=R, o=o, work=work))
-    run = lambda: later.rgsqrf(A, R=R, o=o, work=work)
+    eager_ms = one(lambda: later.rgsqrf(A, R=R, o=o, work=work, hwork=hwork))
+    run = lambda: later.rgsqrf(A, R=R, o=o, work=work, hwork=hwork)
     plan = None
     if graph:
-        plan = later.Plan(A, R, o, work)
+        plan = later.Plan(A, R, o, work, hwork)
         run = plan.execute
     times = []
     for it in range(warmup + steps):
@@ -73,7 +74,7 @@ def time_form(later, torch, A0, form, steps, warmup, graph=True):
             times.append(t)
     if plan is not None:
         plan.close()
-    return times, eager_ms, A, R, work, o
+    return times, eager_ms, A, R, (work, hwork), o
 
 
 def profile_step(later, torch, A0, form, work, o):
@@ -84,7 +85,7 @@ def profile_step(later, torch, A0, form, work, o):
     A.copy_(A0)
     torch.cuda.synchronize()
     later.prof_begin()
-    later.rgsqrf(A, R=R, o=o, work=work)
+    later.rgsqrf(A, R=R, o=o, work=work[0], hwork=work[1])
     return later.prof_end()
 
 

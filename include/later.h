@@ -102,8 +102,12 @@ void later_opts_mixed(later_opts *o);
 /* The plain-FP32 form: FP32 MFMA everywhere, reference panel, one pass. */
 void later_opts_fp32(later_opts *o);
 
-/* Workspace query: FP32 elements of `work` and 16-bit elements of `hwork`
- * that later_rgsqrf_ex needs for (m, n, opts). */
+/* Workspace query: *lwork = FP32 elements of `work` that later_rgsqrf_ex
+ * needs for (m, n, opts); *lhwork = 16-bit elements of `hwork` with which
+ * the node GEMMs of 512+ columns stage their operands as pre-split FP16
+ * pieces (the reference's s2h staging, util/util.cu:24-32) instead of
+ * converting them in the kernel: faster, same result bits; a smaller or
+ * absent hwork selects the in-kernel conversion (0: nothing to stage). */
 int later_rgsqrf_workspace(int m, int n, const later_opts *opts, int64_t *lwork, int64_t *lhwork);
 
 /* Reference entry point, same argument list as include/LATER.h:39 (hwork is
@@ -133,7 +137,7 @@ int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, i
  * with one graph launch and the same effect as the eager call.  The buffers
  * must stay allocated while the plan exists. */
 int later_plan_create(int m, int n, float *A, int lda, float *R, int ldr, float *work, int64_t lwork,
-                      const later_opts *opts, void **plan);
+                      uint16_t *hwork, int64_t lhwork, const later_opts *opts, void **plan);
 int later_plan_execute(void *plan, void *stream);
 int later_plan_kernels(void *plan);
 int later_plan_destroy(void *plan);

@@ -93,7 +93,8 @@ _SIGNATURES = {
     "later_server_stats": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                 ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "later_server_destroy": (_i, [_vp]),
-    "later_plan_create": (_i, [_i, _i, _vp, _i, _vp, _i, _vp, _i64, ctypes.POINTER(later_opts), ctypes.POINTER(_vp)]),
+    "later_plan_create": (_i, [_i, _i, _vp, _i, _vp, _i, _vp, _i64, _vp, _i64, ctypes.POINTER(later_opts),
+                               ctypes.POINTER(_vp)]),
     "later_plan_execute": (_i, [_vp, _vp]),
     "later_plan_kernels": (_i, [_vp]),
     "later_plan_destroy": (_i, [_vp]),
@@ -179,11 +180,17 @@ def _check(rc: int, what: str):
 # --------------------------------------------------------------------------
 # QR
 # --------------------------------------------------------------------------
-def workspace_size(m: int, n: int, o: later_opts | None = None) -> int:
+def workspace_sizes(m: int, n: int, o: later_opts | None = None) -> tuple[int, int]:
+    """(lwork, lhwork): FP32 elements of `work` (required) and 16-bit
+    elements of `hwork` (optional operand staging)."""
     lw, lh = _i64(), _i64()
     _check(lib().later_rgsqrf_workspace(m, n, ctypes.byref(o) if o is not None else None,
                                         ctypes.byref(lw), ctypes.byref(lh)), "later_rgsqrf_workspace")
-    return int(lw.value)
+    return int(lw.value), int(lh.value)
+
+
+def workspace_size(m: int, n: int, o: later_opts | None = None) -> int:
+    return workspace_sizes(m, n, o)[0]
 
 
 def workspace(m: int, n: int, o: later_opts | None = None, device="cuda"):
@@ -191,18 +198,33 @@ def workspace(m: int, n: int, o: later_opts | None = None, device="cuda"):
     return torch.empty(max(workspace_size(m, n, o), 1), device=device, dtype=torch.float32)
 
 
-def rgsqrf(A, R=None, o: later_opts | None = None, work=None, stream=None, info: bool = False):
-    """QR of the column-major FP32 tensor A in place: A <- Q.  Returns R (and
-    the later_info when info=True).  Asynchronous on the current stream."""
+def hworkspace(m: int, n: int, o: later_opts | None = None, device="cuda"):
+    """The 16-bit staging arena of the query (None when nothing is staged)."""
     torch = _torch()
+    lh = workspace_sizes(m, n, o)[1]
+    return torch.empty(lh, device=device, dtype=torch.int16) if lh > 0 else None
+
+
+def _hw(hwork):
+    return (hwork.data_ptr(), int(hwork.numel())) if hwork is not None else (None, 0)
+
+
+def rgsqrf(A, R=None, o: later_opts | None = None, work=None, stream=None, info: bool = False, hwork="auto"):
+    """QR of the column-major FP32 tensor A in place: A <- Q.  Returns R (and
+    the later_info when info=True).  Asynchronous on the current stream.
+    hwork: the 16-bit staging arena ("auto": allocated by the query; None:
+    no staging, the GEMMs convert in the kernel with the same bits)."""
     m, n = A.shape
     if R is None:
         R = colmajor(n, n, device=A.device)
     if work is None:
         work = workspace(m, n, o, device=A.device)
+    if isinstance(hwork, str):
+        hwork = hworkspace(m, n, o, device=A.device)
+    hp, hn = _hw(hwork)
     inf = later_info()
     rc = lib().later_rgsqrf_ex(ctxt(stream), m, n, A.data_ptr(), _ld(A), R.data_ptr(), _ld(R),
-                               work.data_ptr(), work.numel(), None, 0,
+                               work.data_ptr(), work.numel(), hp, hn,
                                ctypes.byref(o) if o is not None else None, ctypes.byref(inf))
     _check(rc, "later_rgsqrf_ex")
     return (R, inf) if info else R
@@ -211,14 +233,16 @@ def rgsqrf(A, R=None, o: later_opts | None = None, work=None, stream=None, info:
 class Plan:
     """A graph-captured factorisation of A (in place) into R for fixed buffers."""
 
-    def __init__(self, A, R, o: later_opts | None = None, work=None):
+    def __init__(self, A, R, o: later_opts | None = None, work=None, hwork="auto"):
         m, n = A.shape
         self.A, self.R = A, R
         self.work = work if work is not None else workspace(m, n, o, device=A.device)
+        self.hwork = hworkspace(m, n, o, device=A.device) if isinstance(hwork, str) else hwork
+        hp, hn = _hw(self.hwork)
         self.o = o
         h = _vp()
         _check(lib().later_plan_create(m, n, A.data_ptr(), _ld(A), R.data_ptr(), _ld(R), self.work.data_ptr(),
-                                       self.work.numel(), ctypes.byref(o) if o is not None else None,
+                                       self.work.numel(), hp, hn, ctypes.byref(o) if o is not None else None,
                                        ctypes.byref(h)), "later_plan_create")
         self.h = h
 

@@ -245,11 +245,11 @@ struct Builder {
             g.C = (float *)p.B; g.ldc = M->lda;
             g.a_exp = aexp; g.b_exp = p.b_exp;
             g.kper = (n1 + 31) / 32 * 32;
-            g.chunk_rows = ch64 ? kFoldAtEnd : 0;
-            g.partial = nullptr; g.update = 1; g.alpha = -1.0f; g.tri = 0;
+            g.chunk_rows = 0;
+            g.partial = nullptr; g.update = ch64 ? 2 : 1; g.alpha = -1.0f; g.tri = 0;
             ua.push_back(g);
         }
-        gemm(update_prec(prec), false, ch64, ua);
+        gemm(update_prec(prec), false, false, ua);
     }
 
     void tree(const std::vector<Mat *> &ms, int c0, int wn, int prec, int chunk, bool pass2)
@@ -302,11 +302,11 @@ struct Builder {
                 g.A = M->R2; g.lda = M->n; g.B = M->T; g.ldb = M->n; g.C = M->R; g.ldc = M->ldr;
                 g.a_exp = scaled(p2) ? kQExp : 0; g.b_exp = scaled(p2) ? M->exps : nullptr;
                 g.kper = (M->n + 31) / 32 * 32;
-                g.chunk_rows = o.chunk > 0 ? kFoldAtEnd : 0;
-                g.partial = nullptr; g.update = 1; g.alpha = 1.0f; g.tri = 1;
+                g.chunk_rows = 0;
+                g.partial = nullptr; g.update = o.chunk > 0 ? 2 : 1; g.alpha = 1.0f; g.tri = 1;
                 ta.push_back(g);
             }
-            gemm(p2, false, o.chunk > 0, ta);
+            gemm(p2, false, false, ta);
         }
     }
 };

@@ -17,13 +17,21 @@ struct GemmArgs {
     int kper;                      // K rows per split (multiple of 64)
     int chunk_rows;                // K rows per FP64 fold, a multiple of 32 (0: no FP64 accumulation)
     void *partial;                 // split-K partials (nsplit x M x N), nullptr: write C
-    int update;                    // 0: C = result; 1: C += alpha * result
+    int update;                    // 0: C = result; 1: C += alpha * result (FP32); 2: the same in FP64
     float alpha;
     int tri;                       // op(A), B upper triangular: skip zero tiles and K ranges
+    // Pre-split operands (nullptr: convert A and B in the kernel): 16-bit
+    // pieces, K-major (row i of op(A) at Ah + i * ldah, piece c at
+    // Ah + c * psa), already scaled by 2^a_exp / 2^b_exp[j].
+    const uint16_t *Ah; int64_t ldah, psa;
+    const uint16_t *Bh; int64_t ldbh, psb;
 };
 
-// chunk_rows that folds into FP64 once, after the last K row.
-constexpr int kFoldAtEnd = 1 << 30;
+// One operand in pre-split form (see gemm_split).
+struct Pieces {
+    const uint16_t *p;
+    int64_t ld, stride;
+};
 
 struct ColexpArgs {
     int m;
@@ -42,22 +50,36 @@ int colexp_grouped(hipStream_t s, const ColexpArgs *args, const int2 *tasks, int
 // Floats of workspace the projection needs for its split-K partials.
 int64_t gemm_proj_work_floats(int m, int n1, int n2, int chunk);
 
+// 16-bit pieces per value of a precision (1, 2 or 3; 0 for FP32).
+int gemm_pieces(int prec);
+
+// Splits X (rows x cols FP32, column-major, ldx), scaled by 2^a_exp (col_exp
+// null) or 2^col_exp[j], into the MFMA input pieces of `prec`: piece c at
+// out + c * stride, column-major with ld `ldo` (trans false: element (i, j)
+// at j * ldo + i) or transposed (element (i, j) at i * ldo + j).
+int gemm_split(hipStream_t s, int prec, int rows, int cols, const float *X, int64_t ldx, int a_exp,
+               const int *col_exp, uint16_t *out, int64_t ldo, int64_t stride, bool trans, int *launches);
+
 // C (n1 x n2) = Q^T B.  a_exp scales Q by 2^a_exp before the cast; b_exp
 // (per column of B, may be null) likewise.  chunk > 0 folds FP32 MFMA
 // accumulators into FP64 every `chunk` rows of K and reduces split-K in FP64.
+// qh, bh (optional): Q and B pre-split (K-major: Q column-major, B
+// column-major), for the same bits without the in-kernel conversion.
 int gemm_proj(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, int64_t ldq, const float *B,
               int64_t ldb, float *C, int64_t ldc, int chunk, const int *b_exp, int a_exp, float *work,
-              int64_t wfloats, int *launches);
+              int64_t wfloats, int *launches, const Pieces *qh = nullptr, const Pieces *bh = nullptr);
 
 // B (m x n2) -= Q C with Q m x n1, C n1 x n2.  f64 subtracts in FP64.
+// qh (Q transposed), ch (C column-major): optional pre-split operands.
 int gemm_update(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, int64_t ldq, const float *C,
-                int64_t ldc, float *B, int64_t ldb, bool f64, const int *c_exp, int a_exp, int *launches);
+                int64_t ldc, float *B, int64_t ldb, bool f64, const int *c_exp, int a_exp, int *launches,
+                const Pieces *qh = nullptr, const Pieces *ch = nullptr);
 
 // C (M x N) += alpha * A B with A M x K (M-contiguous), B K x N.  tri: A and B
 // are upper triangular (C's strictly-lower tiles are left untouched).
 int gemm_nn(hipStream_t s, int prec, int M, int N, int K, float alpha, const float *A, int64_t lda,
             const float *B, int64_t ldb, float *C, int64_t ldc, bool f64, const int *b_exp, int a_exp, bool tri,
-            int *launches);
+            int *launches, const Pieces *ah = nullptr, const Pieces *bh = nullptr);
 
 // e[j] such that max |X(:,j)| 2^e[j] lies in [2^14, 2^15).
 int gemm_colexp(hipStream_t s, int m, int n, const float *X, int64_t ldx, int *e, int *launches);

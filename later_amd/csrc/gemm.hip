@@ -13,6 +13,8 @@
 // With `chunk` set, the FP32 MFMA accumulators are folded into FP64 registers
 // every `chunk` rows of K, and split-K partials are summed in FP64 in a fixed
 // order, so results are bit-reproducible and independent of the launch.
+#include <type_traits>
+
 #include "common.h"
 #include "gemm.h"
 #include "prof.h"
@@ -33,6 +35,15 @@ template <int PREC, bool CH64> constexpr int bk()
     return ((PREC == GEMM_F16X3 || PREC == GEMM_F16X6) && !CH64) ? 64 : 32;
 }
 
+// Slabs of global loads in flight (see the main loop of gemm_body).
+template <int PREC, bool CH64, int BM> constexpr int stages()
+{
+    if (PREC == GEMM_F32) return 2;
+    if (BM == 64) return 4;
+    if (PREC == GEMM_F16X3 && CH64) return 1;  // the FP64 fold's registers leave no room for more
+    return bk<PREC, CH64>() == 64 ? 2 : 3;
+}
+
 template <int PREC> struct Lds;
 template <> struct Lds<GEMM_F32> { typedef float T; };
 template <> struct Lds<GEMM_F16> { typedef _Float16 T; };
@@ -46,15 +57,38 @@ template <> struct Lds<GEMM_F16X6> { typedef _Float16 T; };
 // group g holds sum_{i+j=g} p_i q_j.
 template <int PREC> constexpr int pieces() { return PREC == GEMM_F16X6 ? 3 : PREC == GEMM_F16X3 ? 2 : 1; }
 
+// The MFMA input pieces of one (already scaled) FP32 value.  The fused
+// conversion of the slab stores and the split kernels both use this, so
+// pre-split operands give the MFMAs the same bits.
+template <int PREC>
+__device__ __forceinline__ void split_value(float x, typename Lds<PREC>::T *out)
+{
+    typedef typename Lds<PREC>::T T;
+    if constexpr (pieces<PREC>() == 1) {
+        out[0] = (T)x;
+    } else {
+        const _Float16 h = (_Float16)x;
+        const float r = x - (float)h;  // exact
+        const _Float16 m = (_Float16)(r * 2048.0f);
+        out[0] = h;
+        out[1] = m;
+        if constexpr (pieces<PREC>() == 3) out[2] = (_Float16)((r - (float)m * 0x1p-11f) * 0x1p22f);
+    }
+}
+
 
 
 // Loads of one BK-deep slab of an R-row operand tile into registers.
-//   KM (rows contiguous along K): thread owns SEG consecutive k of one row.
+//   KM (rows contiguous along K): the row's BK values are CPL 16-byte chunks;
+//     consecutive lanes take consecutive chunks of one row, so every wave
+//     load covers whole 128-byte lines (8 rows of 32 k, or 4 rows of 64 k).
+//     A thread owns 4 consecutive k of NPASS rows, RPI rows apart.
 //   !KM (rows contiguous along M): thread owns 4 consecutive rows x KG k.
 template <int R, bool KM, int BK> struct Slab {
     static constexpr int NV = R * BK / NT;
-    static constexpr int SEG = NV;              // KM
-    static constexpr int TPR = BK / SEG;        // KM: threads per row
+    static constexpr int CPL = BK / 4;          // KM: chunks per row
+    static constexpr int RPI = NT / CPL;        // KM: rows per pass of the workgroup
+    static constexpr int NPASS = NV / 4;        // KM: rows per thread
     static constexpr int KG = NV / 4;           // !KM: k per thread
     float v[NV];
 
@@ -66,22 +100,22 @@ template <int R, bool KM, int BK> struct Slab {
     {
         const int t = threadIdx.x;
         if (KM) {
-            const int row = r0 + t / TPR;
-            const int k = k0 + (t % TPR) * SEG;
-            const bool rin = row < rmax;
-            const float *p = P + (int64_t)min(row, rmax - 1) * ld;
-            if (vec && rin && k + SEG <= kmax) {
+            const int k = k0 + 4 * (t % CPL);
 #pragma unroll
-                for (int e = 0; e < SEG; e += 4) {
-                    floatx4 x = *(const floatx4 *)(p + k + e);
-                    v[e] = x[0]; v[e + 1] = x[1]; v[e + 2] = x[2]; v[e + 3] = x[3];
+            for (int p = 0; p < NPASS; ++p) {
+                const int row = r0 + p * RPI + t / CPL;
+                const bool rin = row < rmax;
+                const float *q = P + (int64_t)min(row, rmax - 1) * ld;
+                if (vec && rin && k + 4 <= kmax) {
+                    floatx4 x = *(const floatx4 *)(q + k);
+                    v[4 * p] = x[0]; v[4 * p + 1] = x[1]; v[4 * p + 2] = x[2]; v[4 * p + 3] = x[3];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) v[4 * p + e] = q[min(k + e, kmax - 1)];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (!(rin && k + e < kmax)) v[4 * p + e] = 0.0f;
                 }
-            } else {
-#pragma unroll
-                for (int e = 0; e < SEG; ++e) v[e] = p[min(k + e, kmax - 1)];
-#pragma unroll
-                for (int e = 0; e < SEG; ++e)
-                    if (!(rin && k + e < kmax)) v[e] = 0.0f;
             }
         } else {
             const int rg = r0 + (t % (R / 4)) * 4;
@@ -112,12 +146,12 @@ template <int R, bool KM, int BK> struct Slab {
     __device__ __forceinline__ int row_of(int e) const
     {
         const int t = threadIdx.x;
-        return KM ? t / TPR : (t % (R / 4)) * 4 + (e & 3);
+        return KM ? (e >> 2) * RPI + t / CPL : (t % (R / 4)) * 4 + (e & 3);
     }
     __device__ __forceinline__ int k_of(int e) const
     {
         const int t = threadIdx.x;
-        return KM ? (t % TPR) * SEG + e : (t / (R / 4)) * KG + (e >> 2);
+        return KM ? 4 * (t % CPL) + (e & 3) : (t / (R / 4)) * KG + (e >> 2);
     }
 };
 
@@ -137,33 +171,34 @@ template <typename T, int N> struct Vec {
     typedef T type __attribute__((ext_vector_type(N)));
 };
 
-// Writes a thread's slab to LDS, converted to the MFMA input type, with the
-// widest stores the layout allows: 16-bit types keep k contiguous per row, so
-// a K-major slab (SEG consecutive k of one row) goes out as 16-byte vectors
-// and an M-major slab (KG consecutive k for each of 4 rows) as KG-wide ones;
-// FP32 keeps rows contiguous per k, so an M-major slab writes 4-row vectors.
+// Writes a thread's slab to LDS, converted to the MFMA input type.  A K-major
+// slab holds 4 consecutive k of each of its rows (one 8-byte store per row
+// and piece for the 16-bit types); an M-major slab holds KG consecutive k of
+// each of 4 rows (KG-wide stores for the 16-bit types, 4-row vectors for
+// FP32, whose LDS rows are contiguous along M).  sc: the scale of each row
+// (K-major: one per pass) or of the whole slab (M-major).
 template <int PREC, int R, bool KM, int BK>
-__device__ __forceinline__ void store_slab(Tile<PREC, R, BK> &t, const Slab<R, KM, BK> &s, float scale)
+__device__ __forceinline__ void store_slab(Tile<PREC, R, BK> &t, const Slab<R, KM, BK> &s, const float *sc)
 {
     typedef typename Lds<PREC>::T T;
     typedef Slab<R, KM, BK> S;
+    constexpr int NP = pieces<PREC>();
     if constexpr (PREC == GEMM_F32) {
         if constexpr (KM) {
 #pragma unroll
-            for (int e = 0; e < S::NV; ++e) t.p[0][t.idx(s.row_of(e), s.k_of(e))] = s.v[e] * scale;
+            for (int e = 0; e < S::NV; ++e) t.p[0][t.idx(s.row_of(e), s.k_of(e))] = s.v[e] * sc[e >> 2];
         } else {
 #pragma unroll
             for (int kk = 0; kk < S::KG; ++kk) {
                 floatx4 x;
 #pragma unroll
-                for (int rr = 0; rr < 4; ++rr) x[rr] = s.v[kk * 4 + rr] * scale;
+                for (int rr = 0; rr < 4; ++rr) x[rr] = s.v[kk * 4 + rr] * sc[0];
                 *(floatx4 *)&t.p[0][t.idx(s.row_of(kk * 4), s.k_of(kk * 4))] = x;
             }
         }
     } else {
-        constexpr int W = KM ? 8 : S::KG;      // elements per store
-        constexpr int NST = S::NV / W;          // stores per array
-        constexpr int NP = pieces<PREC>();
+        constexpr int W = KM ? 4 : S::KG;      // elements per store
+        constexpr int NST = S::NV / W;          // stores per piece
         typedef typename Vec<T, W>::type VT;
 #pragma unroll
         for (int q = 0; q < NST; ++q) {
@@ -171,26 +206,68 @@ __device__ __forceinline__ void store_slab(Tile<PREC, R, BK> &t, const Slab<R, K
             int row = 0, k = 0;
 #pragma unroll
             for (int e = 0; e < W; ++e) {
-                // KM: values q*W .. q*W+W-1 are consecutive k of one row.
+                // K-major: values 4q .. 4q+3 are consecutive k of row q.
                 // M-major: value kk*4+rr is row rr, k kk; store q = row rr.
                 const int idx = KM ? q * W + e : e * 4 + q;
-                float x = s.v[idx] * scale;
-                if constexpr (NP == 1) {
-                    pv[0][e] = (T)x;
-                } else {
-                    const _Float16 h = (_Float16)x;
-                    const float r = x - (float)h;  // exact
-                    const _Float16 m = (_Float16)(r * 2048.0f);
-                    pv[0][e] = h;
-                    pv[1][e] = m;
-                    if constexpr (NP == 3) pv[2][e] = (_Float16)((r - (float)m * 0x1p-11f) * 0x1p22f);
-                }
+                T pc[NP];
+                split_value<PREC>(s.v[idx] * (KM ? sc[q] : sc[0]), pc);
+#pragma unroll
+                for (int c = 0; c < NP; ++c) pv[c][e] = pc[c];
                 if (e == 0) { row = s.row_of(idx); k = s.k_of(idx); }
             }
 #pragma unroll
             for (int c = 0; c < NP; ++c) *(VT *)&t.p[c][t.idx(row, k)] = pv[c];
         }
     }
+}
+
+typedef unsigned short ushort8 __attribute__((ext_vector_type(8)));
+
+// Loads of one BK-deep slab of pre-split 16-bit pieces (K-major, piece c at
+// P + c * piece): consecutive lanes take consecutive 16-byte chunks (8 k) of
+// one row.  A thread owns one chunk of NPASS rows, RPI rows apart.
+template <int R, int BK, int NP> struct Slab16 {
+    static constexpr int CPL = BK / 8;      // chunks per row
+    static constexpr int RPI = NT / CPL;    // rows per pass of the workgroup
+    static constexpr int NPASS = R / RPI;
+    ushort8 v[NP][NPASS];
+
+    __device__ __forceinline__ void load(const uint16_t *P, int64_t ld, int64_t piece, int r0, int rmax, int k0,
+                                         int kmax, bool vec)
+    {
+        const int t = threadIdx.x;
+        const int k = k0 + 8 * (t % CPL);
+#pragma unroll
+        for (int p = 0; p < NPASS; ++p) {
+            const int row = r0 + p * RPI + t / CPL;
+            const bool rin = row < rmax;
+#pragma unroll
+            for (int c = 0; c < NP; ++c) {
+                const uint16_t *q = P + c * piece + (int64_t)min(row, rmax - 1) * ld;
+                if (vec && rin && k + 8 <= kmax) {
+                    v[c][p] = *(const ushort8 *)(q + k);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const unsigned short u = q[min(k + e, kmax - 1)];
+                        v[c][p][e] = (rin && k + e < kmax) ? u : (unsigned short)0;
+                    }
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ int row_of(int p) const { return p * RPI + (int)threadIdx.x / CPL; }
+    __device__ __forceinline__ int k_of() const { return 8 * ((int)threadIdx.x % CPL); }
+};
+
+template <int PREC, int R, int BK, int NP>
+__device__ __forceinline__ void store_slab16(Tile<PREC, R, BK> &t, const Slab16<R, BK, NP> &s)
+{
+#pragma unroll
+    for (int c = 0; c < NP; ++c)
+#pragma unroll
+        for (int p = 0; p < Slab16<R, BK, NP>::NPASS; ++p)
+            *(ushort8 *)&t.p[c][t.idx(s.row_of(p), s.k_of())] = s.v[c][p];
 }
 
 template <int PREC> struct Frag;
@@ -223,7 +300,9 @@ __device__ __forceinline__ typename Frag<PREC>::T frag(const typename Lds<PREC>:
     }
 }
 
-template <int PREC, int BM, int BN, bool AKM, bool CH64>
+// PRE: both operands come pre-split (GemmArgs.Ah / Bh, K-major 16-bit
+// pieces); otherwise they are read as FP32 and converted on the way to LDS.
+template <int PREC, int BM, int BN, bool AKM, bool CH64, bool PRE>
 __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const int tn, const int split)
 {
     constexpr int WM = BM / 2, WN = BN / 2;     // wave tile
@@ -256,11 +335,34 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
 
     // Operand scales (powers of two, so exact).  A has one global scale; a
     // thread's B values all lie in one column of B (B rows are K-contiguous).
-    Slab<BM, AKM, BK> sa;
-    Slab<BN, true, BK> sb;
-    const float ascale = ldexpf(1.0f, g.a_exp);
-    const int bcol = n0 + sb.row_of(0);
-    const float bscale = (g.b_exp && bcol < g.N) ? ldexpf(1.0f, g.b_exp[bcol]) : 1.0f;
+    constexpr int S = stages<PREC, CH64, BM>();
+    typedef typename std::conditional<PRE, Slab16<BM, BK, NP>, Slab<BM, AKM, BK>>::type SA;
+    typedef typename std::conditional<PRE, Slab16<BN, BK, NP>, Slab<BN, true, BK>>::type SB;
+    SA sa[S];
+    SB sb[S];
+    // Scales of the rows a thread converts: A has one; B's rows are columns
+    // of B with one exponent each.  (Pre-split pieces are already scaled.)
+    constexpr int NSA = PRE ? 1 : (AKM ? SA::NPASS : 1), NSB = PRE ? 1 : SB::NPASS;
+    float ascale[NSA], bscale[NSB];
+#pragma unroll
+    for (int p = 0; p < NSA; ++p) ascale[p] = ldexpf(1.0f, g.a_exp);
+#pragma unroll
+    for (int p = 0; p < NSB; ++p) {
+        int bcol = 0;
+        if constexpr (!PRE) bcol = n0 + sb[0].row_of(4 * p);
+        bscale[p] = (g.b_exp && bcol < g.N) ? ldexpf(1.0f, g.b_exp[bcol]) : 1.0f;
+    }
+    const bool avec16 = PRE && ((g.ldah & 7) == 0) && ((((uintptr_t)g.Ah) & 15) == 0) && ((g.psa & 7) == 0);
+    const bool bvec16 = PRE && ((g.ldbh & 7) == 0) && ((((uintptr_t)g.Bh) & 15) == 0) && ((g.psb & 7) == 0);
+    auto store = [&](int buf, const SA &a, const SB &b) {
+        if constexpr (PRE) {
+            store_slab16<PREC, BM, BK, NP>(As[buf], a);
+            store_slab16<PREC, BN, BK, NP>(Bs[buf], b);
+        } else {
+            store_slab<PREC, BM, AKM, BK>(As[buf], a, ascale);
+            store_slab<PREC, BN, true, BK>(Bs[buf], b, bscale);
+        }
+    };
 
     // The MFMA computes the transposed tile (B operand rows x A operand rows),
     // so that lane l%32 walks the rows of C: stores of column-major C are
@@ -307,21 +409,9 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
     };
 
     const int ntiles = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
-    if (ntiles > 0) {
-        sa.load(g.A, g.lda, m0, g.M, kbeg, kend, avec);
-        sb.load(g.B, g.ldb, n0, g.N, kbeg, kend, bvec);
-        store_slab<PREC, BM, AKM, BK>(As[0], sa, ascale);
-        store_slab<PREC, BN, true, BK>(Bs[0], sb, bscale);
-        __syncthreads();
-    }
-    for (int it = 0; it < ntiles; ++it) {
-        const int cur = it & 1;
-        const bool more = it + 1 < ntiles;
-        if (more) {
-            const int k0 = kbeg + (it + 1) * BK;
-            sa.load(g.A, g.lda, m0, g.M, k0, kend, avec);
-            sb.load(g.B, g.ldb, n0, g.N, k0, kend, bvec);
-        }
+
+    // One slab of MFMAs from LDS buffer `cur` (slab `it` of this split).
+    auto compute = [&](int cur, int it) {
 #pragma unroll
         for (int ks = 0; ks < KSTEPS; ++ks) {
             typename Frag<PREC>::T fa[NP][FM], fb[NP][FN];
@@ -348,11 +438,42 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
             if (CH64 && ks + 1 < KSTEPS && (it * BK + (ks + 1) * KR) % g.chunk_rows == 0) fold();
         }
         if (CH64 && (((it + 1) * BK) % g.chunk_rows == 0 || it + 1 == ntiles)) fold();
-        if (more) {
-            store_slab<PREC, BM, AKM, BK>(As[cur ^ 1], sa, ascale);
-            store_slab<PREC, BN, true, BK>(Bs[cur ^ 1], sb, bscale);
+    };
+    auto load = [&](int slot_it, SA &a, SB &b) {
+        const int k0 = kbeg + slot_it * BK;
+        if constexpr (PRE) {
+            a.load(g.Ah, g.ldah, g.psa, m0, g.M, k0, kend, avec16);
+            b.load(g.Bh, g.ldbh, g.psb, n0, g.N, k0, kend, bvec16);
+        } else {
+            a.load(g.A, g.lda, m0, g.M, k0, kend, avec);
+            b.load(g.B, g.ldb, n0, g.N, k0, kend, bvec);
         }
+    };
+
+    // Global loads run S slabs ahead of the MFMAs through a ring of register
+    // slabs: slab j lives in slot j % S from its load until it is written to
+    // LDS, S iterations later, so S slabs of compute cover its latency.
+    // LDS itself is double-buffered (one barrier per slab).
+    if (ntiles > 0) {
+#pragma unroll
+        for (int u = 0; u < S; ++u)
+            if (u < ntiles) load(u, sa[u], sb[u]);
+        store(0, sa[0], sb[0]);
         __syncthreads();
+        if (S < ntiles) load(S, sa[0], sb[0]);
+    }
+    for (int it0 = 0; it0 < ntiles; it0 += S) {
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int it = it0 + u;
+            if (it >= ntiles) break;
+            const int cur = it & 1;
+            compute(cur, it);
+            const int nxt = (u + 1) % S;  // static once the loop over u is unrolled
+            if (it + 1 < ntiles) store(cur ^ 1, sa[nxt], sb[nxt]);
+            __syncthreads();
+            if (it + 1 + S < ntiles) load(it + 1 + S, sa[nxt], sb[nxt]);
+        }
     }
 
     // Epilogue.  Transposed 32x32 accumulator layout: lane l holds row l%32
@@ -379,7 +500,7 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
                     else ((float *)g.partial)[off] = (float)v;
                 } else if (g.update) {
                     float *c = g.C + (int64_t)col * g.ldc + row;
-                    if constexpr (CH64) *c = (float)((double)*c + (double)g.alpha * v);
+                    if (CH64 || g.update == 2) *c = (float)((double)*c + (double)g.alpha * v);
                     else *c = fmaf(g.alpha, (float)v, *c);
                 } else {
                     g.C[(int64_t)col * g.ldc + row] = (float)v;
@@ -394,7 +515,7 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
 // through a contiguous range, and each range walks the tiles in groups of 8
 // tile-rows (column by column inside a group), so the workgroups resident on
 // one XCD share their A and B tiles in its L2.
-template <int PREC, int BM, int BN, bool AKM, bool CH64>
+template <int PREC, int BM, int BN, bool AKM, bool CH64, bool PRE>
 __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
 {
     constexpr int kXcd = 8, kGroup = 8;
@@ -409,7 +530,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
     const int first = (r / (kGroup * ny)) * kGroup;
     const int gsz = min(nx - first, kGroup);
     const int rr = r - first * ny;
-    gemm_body<PREC, BM, BN, AKM, CH64>(g, first + rr % gsz, rr / gsz, split);
+    gemm_body<PREC, BM, BN, AKM, CH64, PRE>(g, first + rr % gsz, rr / gsz, split);
 }
 
 // Grouped launch: workgroup b computes tile (t.y, t.z) of split t.w of the
@@ -419,7 +540,7 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const GemmArgs *args, 
 {
     const int4 t = tasks[blockIdx.x];
     const GemmArgs g = args[t.x];
-    gemm_body<PREC, BM, BN, AKM, CH64>(g, t.y, t.z, t.w);
+    gemm_body<PREC, BM, BN, AKM, CH64, false>(g, t.y, t.z, t.w);
 }
 
 // C = sum over splits (fixed order, FP64) of the partials.
@@ -479,37 +600,120 @@ __global__ void colexp_grouped_kernel(const ColexpArgs *args, const int2 *tasks)
     }
 }
 
-template <int PREC, int BM, int BN, bool AKM, bool CH64>
-void launch_one(hipStream_t s, const GemmArgs &g, dim3 grid)
+// Pieces of X scaled by 2^e (e = a_exp or col_exp[j]), same layout: each
+// thread converts 4 consecutive rows of one column.
+template <int PREC>
+__global__ void split_kernel(int rows, int cols, const float *X, int64_t ldx, int a_exp, const int *col_exp,
+                             uint16_t *out, int64_t ldo, int64_t stride)
 {
-    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, AKM, CH64>), grid, dim3(NT), 0, s, g);
+    typedef typename Lds<PREC>::T T;
+    constexpr int NP = pieces<PREC>();
+    const int j = blockIdx.y;
+    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i0 >= rows) return;
+    const float sc = ldexpf(1.0f, col_exp ? col_exp[j] : a_exp);
+    const float *x = X + (int64_t)j * ldx;
+    uint16_t *o = out + (int64_t)j * ldo;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int i = i0 + e;
+        if (i < rows) {
+            T pc[NP];
+            split_value<PREC>(x[i] * sc, pc);
+#pragma unroll
+            for (int c = 0; c < NP; ++c) o[c * stride + i] = __builtin_bit_cast(uint16_t, pc[c]);
+        }
+    }
 }
 
-template <int PREC, bool AKM, bool CH64>
+// Transposed pieces: element (i, j) of X goes to out + i * ldo + j.  64 x 64
+// tiles through LDS, so both the FP32 reads (along i) and the 16-bit writes
+// (along j) are contiguous.
+template <int PREC>
+__global__ __launch_bounds__(256) void split_t_kernel(int rows, int cols, const float *X, int64_t ldx, int a_exp,
+                                                     const int *col_exp, uint16_t *out, int64_t ldo, int64_t stride)
+{
+    typedef typename Lds<PREC>::T T;
+    constexpr int NP = pieces<PREC>();
+    __shared__ float tile[64][65];
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int jj = ty + 4 * q, i = i0 + tx, j = j0 + jj;
+        float v = 0.0f;
+        if (i < rows && j < cols) v = X[i + (int64_t)j * ldx] * ldexpf(1.0f, col_exp ? col_exp[j] : a_exp);
+        tile[jj][tx] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int ii = ty + 4 * q, i = i0 + ii, j = j0 + tx;
+        if (i < rows && j < cols) {
+            T pc[NP];
+            split_value<PREC>(tile[tx][ii], pc);
+#pragma unroll
+            for (int c = 0; c < NP; ++c) out[c * stride + (int64_t)i * ldo + j] = __builtin_bit_cast(uint16_t, pc[c]);
+        }
+    }
+}
+
+template <int PREC>
+void launch_split(hipStream_t s, int rows, int cols, const float *X, int64_t ldx, int a_exp, const int *col_exp,
+                  uint16_t *out, int64_t ldo, int64_t stride, bool trans)
+{
+    if (trans) {
+        dim3 grid((unsigned)ceil_div64(rows, 64), (unsigned)ceil_div64(cols, 64));
+        hipLaunchKernelGGL(split_t_kernel<PREC>, grid, dim3(256), 0, s, rows, cols, X, ldx, a_exp, col_exp, out, ldo,
+                           stride);
+    } else {
+        dim3 grid((unsigned)ceil_div64(rows, 1024), (unsigned)cols);
+        hipLaunchKernelGGL(split_kernel<PREC>, grid, dim3(256), 0, s, rows, cols, X, ldx, a_exp, col_exp, out, ldo,
+                           stride);
+    }
+}
+
+template <int PREC, int BM, int BN, bool AKM, bool CH64, bool PRE>
+void launch_one(hipStream_t s, const GemmArgs &g, dim3 grid)
+{
+    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, AKM, CH64, PRE>), grid, dim3(NT), 0, s, g);
+}
+
+template <int PREC, bool AKM, bool CH64, bool PRE>
 void launch_tile(hipStream_t s, const GemmArgs &g, int nsplit, bool small)
 {
     const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
     if constexpr (PREC == GEMM_F16X6) {
         // Three accumulator groups: a 128x128 tile would spill, so the
         // three-piece form always runs 64x64 tiles.
-        launch_one<PREC, 64, 64, AKM, CH64>(s, g, g64);
+        launch_one<PREC, 64, 64, AKM, CH64, PRE>(s, g, g64);
     } else if (small) {
-        launch_one<PREC, 64, 64, AKM, CH64>(s, g, g64);
+        launch_one<PREC, 64, 64, AKM, CH64, PRE>(s, g, g64);
     } else {
         dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
-        launch_one<PREC, 128, 128, AKM, CH64>(s, g, grid);
+        launch_one<PREC, 128, 128, AKM, CH64, PRE>(s, g, grid);
     }
 }
 
+// Pre-split operands are K-major on both sides, so those kernels exist in
+// the AKM = true form only.
 template <bool AKM, bool CH64>
 int dispatch(hipStream_t s, int prec, const GemmArgs &g, int nsplit, bool small)
 {
+    const bool pre = g.Ah && g.Bh;
+    if (pre && prec == GEMM_F32) return LATER_ERR_OPTION;
     switch (prec) {
-    case GEMM_F32: launch_tile<GEMM_F32, AKM, CH64>(s, g, nsplit, small); break;
-    case GEMM_F16: launch_tile<GEMM_F16, AKM, CH64>(s, g, nsplit, small); break;
-    case GEMM_BF16: launch_tile<GEMM_BF16, AKM, CH64>(s, g, nsplit, small); break;
-    case GEMM_F16X3: launch_tile<GEMM_F16X3, AKM, CH64>(s, g, nsplit, small); break;
-    case GEMM_F16X6: launch_tile<GEMM_F16X6, AKM, CH64>(s, g, nsplit, small); break;
+    case GEMM_F32: launch_tile<GEMM_F32, AKM, CH64, false>(s, g, nsplit, small); break;
+#define LATER_DISPATCH_16(P)                                                            \
+    case P:                                                                             \
+        if (pre) launch_tile<P, true, CH64, true>(s, g, nsplit, small);                 \
+        else launch_tile<P, AKM, CH64, false>(s, g, nsplit, small);                     \
+        break;
+    LATER_DISPATCH_16(GEMM_F16)
+    LATER_DISPATCH_16(GEMM_BF16)
+    LATER_DISPATCH_16(GEMM_F16X3)
+    LATER_DISPATCH_16(GEMM_F16X6)
+#undef LATER_DISPATCH_16
     default: return LATER_ERR_OPTION;
     }
     HIP_OK(hipGetLastError());
@@ -545,8 +749,8 @@ int gemm_grouped(hipStream_t s, int prec, bool proj, bool ch64, const GemmArgs *
     if (ntasks <= 0) return LATER_OK;
     if (proj) return ch64 ? launch_grouped_p<true, true>(s, prec, args, tasks, ntasks)
                           : launch_grouped_p<true, false>(s, prec, args, tasks, ntasks);
-    return ch64 ? launch_grouped_p<false, true>(s, prec, args, tasks, ntasks)
-                : launch_grouped_p<false, false>(s, prec, args, tasks, ntasks);
+    (void)ch64;  // updates accumulate in FP32 and subtract in FP64 when GemmArgs.update == 2
+    return launch_grouped_p<false, false>(s, prec, args, tasks, ntasks);
 }
 
 int colexp_grouped(hipStream_t s, const ColexpArgs *args, const int2 *tasks, int ntasks)
@@ -589,9 +793,20 @@ int64_t gemm_proj_work_floats(int m, int n1, int n2, int chunk)
     return ns > 1 ? (int64_t)ns * n1 * n2 * (chunk > 0 ? 2 : 1) : 0;
 }
 
+static void set_pieces(GemmArgs &g, const Pieces *ah, const Pieces *bh)
+{
+    const bool pre = ah && bh && ah->p && bh->p;
+    g.Ah = pre ? ah->p : nullptr;
+    g.ldah = pre ? ah->ld : 0;
+    g.psa = pre ? ah->stride : 0;
+    g.Bh = pre ? bh->p : nullptr;
+    g.ldbh = pre ? bh->ld : 0;
+    g.psb = pre ? bh->stride : 0;
+}
+
 int gemm_proj(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, int64_t ldq, const float *B,
               int64_t ldb, float *C, int64_t ldc, int chunk, const int *b_exp, int a_exp, float *work,
-              int64_t wfloats, int *launches)
+              int64_t wfloats, int *launches, const Pieces *qh, const Pieces *bh)
 {
     const bool ch64 = chunk > 0;
     if (ch64 && (chunk % 32 || (chunk < kMaxBK && kMaxBK % chunk))) return LATER_ERR_OPTION;
@@ -609,6 +824,7 @@ int gemm_proj(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, in
     g.update = 0;
     g.alpha = 1.0f;
     g.tri = 0;
+    set_pieces(g, qh, bh);
     const int tok = prof_start(s);
     int rc = ch64 ? dispatch<true, true>(s, prec, g, ns, small) : dispatch<true, false>(s, prec, g, ns, small);
     prof_stop(tok, s, PROF_GEMM_PROJ, 2.0 * m * n1 * n2, 4.0 * ((double)m * (n1 + n2) + (double)n1 * n2));
@@ -626,28 +842,30 @@ int gemm_proj(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, in
 }
 
 int gemm_update(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, int64_t ldq, const float *Cm,
-                int64_t ldc, float *B, int64_t ldb, bool f64, const int *c_exp, int a_exp, int *launches)
+                int64_t ldc, float *B, int64_t ldb, bool f64, const int *c_exp, int a_exp, int *launches,
+                const Pieces *qh, const Pieces *ch)
 {
-    return gemm_nn(s, prec, m, n2, n1, -1.0f, Q, ldq, Cm, ldc, B, ldb, f64, c_exp, a_exp, false, launches);
+    return gemm_nn(s, prec, m, n2, n1, -1.0f, Q, ldq, Cm, ldc, B, ldb, f64, c_exp, a_exp, false, launches, qh, ch);
 }
 
 int gemm_nn(hipStream_t s, int prec, int M, int N, int K, float alpha, const float *A, int64_t lda,
             const float *B, int64_t ldb, float *C, int64_t ldc, bool f64, const int *b_exp, int a_exp, bool tri,
-            int *launches)
+            int *launches, const Pieces *ah, const Pieces *bh)
 {
     GemmArgs g;
     g.M = M; g.N = N; g.K = K;
     g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
     g.a_exp = a_exp; g.b_exp = b_exp;
     g.kper = K;
-    g.chunk_rows = f64 ? kFoldAtEnd : 0;
+    g.chunk_rows = 0;
     g.partial = nullptr;
-    g.update = 1;
+    g.update = f64 ? 2 : 1;
     g.alpha = alpha;
     g.tri = tri ? 1 : 0;
+    set_pieces(g, ah, bh);
     const bool small = (N <= 64);
     const int tok = prof_start(s);
-    int rc = f64 ? dispatch<false, true>(s, prec, g, 1, small) : dispatch<false, false>(s, prec, g, 1, small);
+    int rc = dispatch<false, false>(s, prec, g, 1, small);
     prof_stop(tok, s, PROF_GEMM_UPDATE, tri ? (double)M * N * K / 3.0 : 2.0 * M * N * K,
               4.0 * ((double)M * K + 2.0 * M * N + (double)K * N));
     if (rc) return rc;
@@ -659,6 +877,32 @@ int gemm_colexp(hipStream_t s, int m, int n, const float *X, int64_t ldx, int *e
 {
     if (n <= 0) return LATER_OK;
     hipLaunchKernelGGL(colexp_kernel, dim3(n), dim3(64), 0, s, m, n, X, ldx, e);
+    HIP_OK(hipGetLastError());
+    if (launches) ++*launches;
+    return LATER_OK;
+}
+
+int gemm_pieces(int prec)
+{
+    switch (prec) {
+    case GEMM_F16: case GEMM_BF16: return 1;
+    case GEMM_F16X3: return 2;
+    case GEMM_F16X6: return 3;
+    default: return 0;
+    }
+}
+
+int gemm_split(hipStream_t s, int prec, int rows, int cols, const float *X, int64_t ldx, int a_exp,
+               const int *col_exp, uint16_t *out, int64_t ldo, int64_t stride, bool trans, int *launches)
+{
+    if (rows <= 0 || cols <= 0) return LATER_OK;
+    switch (prec) {
+    case GEMM_F16: launch_split<GEMM_F16>(s, rows, cols, X, ldx, a_exp, col_exp, out, ldo, stride, trans); break;
+    case GEMM_BF16: launch_split<GEMM_BF16>(s, rows, cols, X, ldx, a_exp, col_exp, out, ldo, stride, trans); break;
+    case GEMM_F16X3: launch_split<GEMM_F16X3>(s, rows, cols, X, ldx, a_exp, col_exp, out, ldo, stride, trans); break;
+    case GEMM_F16X6: launch_split<GEMM_F16X6>(s, rows, cols, X, ldx, a_exp, col_exp, out, ldo, stride, trans); break;
+    default: return LATER_ERR_OPTION;
+    }
     HIP_OK(hipGetLastError());
     if (launches) ++*launches;
     return LATER_OK;

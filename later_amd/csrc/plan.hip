@@ -16,7 +16,7 @@ struct Plan {
 } // namespace
 
 LATER_API int later_plan_create(int m, int n, float *A, int lda, float *R, int ldr, float *work, int64_t lwork,
-                                const later_opts *opts, void **plan)
+                                uint16_t *hwork, int64_t lhwork, const later_opts *opts, void **plan)
 {
     if (!plan) return LATER_ERR_SHAPE;
     *plan = nullptr;
@@ -28,7 +28,7 @@ LATER_API int later_plan_create(int m, int n, float *A, int lda, float *R, int l
     later_info info;
     if (!rc) {
         laterCtxt c{(void *)s, nullptr};
-        rc = later_rgsqrf_ex(c, m, n, A, lda, R, ldr, work, lwork, nullptr, 0, opts, &info);
+        rc = later_rgsqrf_ex(c, m, n, A, lda, R, ldr, work, lwork, hwork, lhwork, opts, &info);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s, &g);
         if (!rc && (e != hipSuccess || !g)) rc = LATER_ERR_DEVICE;

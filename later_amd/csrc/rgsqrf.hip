@@ -35,7 +35,33 @@ int update_prec(int p) { return p == LATER_GEMM_F16X6 ? LATER_GEMM_F16X3 : p; }
 
 struct Layout {
     int64_t header, exps, panel, gemm, snode, r2, tcopy, total;
+    int64_t hpieces;  // 16-bit elements of `hwork` for the pre-split operand pieces
 };
+
+// Nodes at least this wide stage their 16-bit GEMM operands as pre-split
+// pieces (below it the four extra launches cost more than they save).
+constexpr int kPieceMinN = 512;
+
+// Leading dimension of a piece array: a multiple of 8 (16-byte rows), kept
+// off powers of two so that the rows of one tile spread over the memory
+// channels.
+int64_t piece_ld(int64_t x) { return (x + 7) / 8 * 8 + 64; }
+
+// Pieces of the projection operands: not for the three-piece form, whose
+// 6 bytes per element cost more to stream than the in-kernel split (measured
+// at the 16384^2 top node).
+int proj_pieces(int prec) { return prec == LATER_GEMM_F16X6 ? 0 : gemm_pieces(prec); }
+
+// 16-bit elements the pre-split operands of one node need: Q1 (column-major
+// and transposed), A2 and R12.
+int64_t node_pieces(int m, int n, int prec)
+{
+    const int np = proj_pieces(prec), npu = gemm_pieces(update_prec(prec));
+    if (npu == 0 || n < kPieceMinN) return 0;
+    const int n1 = split_n(n), n2 = n - n1;
+    const int64_t ldm = piece_ld(m), ldn1 = piece_ld(n1);
+    return np * ldm * (n1 + n2) + npu * ldn1 * ((int64_t)m + n2);
+}
 
 // Largest split-K partial buffer over the nodes of the recursion.
 void walk_gemm(int m, int n, int nmin, int chunk_big, int64_t *mx)
@@ -64,6 +90,13 @@ Layout layout(int m, int n, const later_opts &o)
     L.r2 = (o.reorth & LATER_REORTH_PASS) ? (int64_t)n * n : 0;
     L.tcopy = L.r2;
     L.total = L.header + L.exps + L.panel + L.gemm + L.snode + L.r2 + L.tcopy;
+    // The top node is the largest.
+    int64_t ph = (o.gemm != LATER_GEMM_F32 && n > o.nmin) ? node_pieces(m, n, o.gemm) : 0;
+    if ((o.reorth & LATER_REORTH_PASS) && o.gemm_pass2 != LATER_GEMM_F32 && n > o.nmin) {
+        const int64_t p2 = node_pieces(m, n, o.gemm_pass2);
+        if (p2 > ph) ph = p2;
+    }
+    L.hpieces = ph;
     return L;
 }
 
@@ -75,6 +108,7 @@ struct Run {
     float *panel; int64_t panel_n;
     float *gemm; int64_t gemm_n;
     float *snode;
+    uint16_t *pcs;     // pre-split operand pieces (nullptr: fused conversion)
     int launches;
     int depth;
     bool node_reorth;  // block CGS2 at the nodes of this pass
@@ -139,19 +173,49 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     float *R12 = R + (int64_t)n1 * ldr;
     const int aexp = scaled(p) ? kQExp : 0;
     const int *bexp = nullptr;
+    // Pre-split operands (the reference's s2h staging, util/util.cu:24-32,
+    // as FP16 pieces): the same MFMA inputs as the fused conversion, made
+    // once per node instead of once per tile.
+    const bool pre = r.pcs && !inner && n >= kPieceMinN && gemm_pieces(update_prec(p)) > 0;
+    const bool pre_proj = pre && proj_pieces(p) > 0;
+    Pieces q1h{}, a2h{}, q1t{}, r12h{};
+    if (pre) {
+        const int64_t ldm = piece_ld(m), ldn1 = piece_ld(n1);
+        uint16_t *b = r.pcs;
+        q1h = {b, ldm, ldm * n1}; b += proj_pieces(p) * ldm * n1;
+        a2h = {b, ldm, ldm * n2}; b += proj_pieces(p) * ldm * n2;
+        q1t = {b, ldn1, ldn1 * m}; b += gemm_pieces(update_prec(p)) * ldn1 * m;
+        r12h = {b, ldn1, ldn1 * n2};
+    }
+    if (pre_proj && (rc = gemm_split(r.s, p, m, n1, A, lda, aexp, nullptr, (uint16_t *)q1h.p, q1h.ld, q1h.stride,
+                                     false, &r.launches)))
+        return rc;
     // R12 = Q1^T A2   (QR/later_rgsqrf.cu:43-49)
     if (scaled(p)) {
         if ((rc = gemm_colexp(r.s, m, n2, A2, lda, r.exps, &r.launches))) return rc;
         bexp = r.exps;
     }
+    if (pre_proj && (rc = gemm_split(r.s, p, m, n2, A2, lda, aexp, bexp, (uint16_t *)a2h.p, a2h.ld, a2h.stride,
+                                     false, &r.launches)))
+        return rc;
     rc = gemm_proj(r.s, p, m, n1, n2, A, lda, A2, lda, R12, ldr, chunk, bexp, aexp, r.gemm, r.gemm_n,
-                   &r.launches);
+                   &r.launches, pre_proj ? &q1h : nullptr, pre_proj ? &a2h : nullptr);
     if (rc) return rc;
     // A2 -= Q1 R12    (QR/later_rgsqrf.cu:51-56)
     if (scaled(p)) {
         if ((rc = gemm_colexp(r.s, n1, n2, R12, ldr, r.exps, &r.launches))) return rc;
     }
-    rc = gemm_update(r.s, update_prec(p), m, n1, n2, A, lda, R12, ldr, A2, lda, f64, bexp, aexp, &r.launches);
+    if (pre) {
+        const int pu = update_prec(p);
+        if ((rc = gemm_split(r.s, pu, m, n1, A, lda, aexp, nullptr, (uint16_t *)q1t.p, q1t.ld, q1t.stride, true,
+                             &r.launches)))
+            return rc;
+        if ((rc = gemm_split(r.s, pu, n1, n2, R12, ldr, 0, bexp, (uint16_t *)r12h.p, r12h.ld, r12h.stride, false,
+                             &r.launches)))
+            return rc;
+    }
+    rc = gemm_update(r.s, update_prec(p), m, n1, n2, A, lda, R12, ldr, A2, lda, f64, bexp, aexp, &r.launches,
+                     pre ? &q1t : nullptr, pre ? &r12h : nullptr);
     if (rc) return rc;
     if (r.node_reorth) {
         // S = Q1^T A2; A2 -= Q1 S; R12 += S
@@ -226,7 +290,7 @@ LATER_API int later_rgsqrf_workspace(int m, int n, const later_opts *opts, int64
     if (m < 1 || n < 1 || m < n) return LATER_ERR_SHAPE;
     Layout L = layout(m, n, o);
     if (lwork) *lwork = L.total;
-    if (lhwork) *lhwork = 0;
+    if (lhwork) *lhwork = L.hpieces;
     return LATER_OK;
 }
 
@@ -246,11 +310,14 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     Layout L = layout(m > 0 ? m : 1, n > 0 ? n : 1, o);
     // Everything lives in `work` when it is large enough; otherwise the
     // split-K partials may use the 16-bit arena `hwork` (the reference sizes
-    // it at m*n halves, test/test_qr.cu:76, and this library has no FP16
-    // staging buffers to keep there).
+    // it at m*n halves, test/test_qr.cu:76).
     const bool partial_in_h = work && lwork < L.total && lwork >= L.total - L.gemm && hwork &&
                               lhwork / 2 >= L.gemm;
     if (!rc && (!work || (lwork < L.total && !partial_in_h))) rc = LATER_ERR_WORKSPACE;
+    // The 16-bit arena holds the pre-split GEMM operands when it is large
+    // enough (and not used for partials); otherwise the GEMMs convert their
+    // operands in the kernel, with the same result bits.
+    const bool with_pieces = !partial_in_h && hwork && L.hpieces > 0 && lhwork >= L.hpieces;
     if (!rc) {
         const int64_t abytes = ((int64_t)(n - 1) * lda + m) * 4, rbytes = ((int64_t)(n - 1) * ldr + n) * 4;
         const int64_t wbytes = lwork * 4, hbytes = lhwork * 2;
@@ -280,6 +347,7 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     r.snode = p; p += L.snode;
     float *R2 = p; p += L.r2;
     float *T = p; p += L.tcopy;
+    r.pcs = with_pieces ? hwork : nullptr;
     r.launches = 0;
     r.depth = 0;
     r.node_reorth = (o.reorth & LATER_REORTH_NODE) != 0;

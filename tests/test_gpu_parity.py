@@ -221,6 +221,33 @@ def test_rgsqrf_deterministic(later, oracle):
         assert np.array_equal(Q1, Q2) and np.array_equal(R1, R2), form
 
 
+@pytest.mark.parametrize("form,m,n", [("reference", 4096, 1024), ("mixed", 3000, 1100), ("fp32", 2048, 1024),
+                                     ("mixed", 2048, 2048)])
+def test_presplit_pieces_give_fused_bits(later, oracle, form, m, n):
+    """Nodes of 512+ columns stage their 16-bit operands as pre-split pieces
+    in the 16-bit arena `hwork` when it is as large as the query asks; one
+    element short of it (or without it), the GEMMs convert in the kernel.
+    Both give the same bits."""
+    A = oracle.gen_uniform(m, n, 0, 0)
+    o = later.opts(form)
+    lw, lh = later.workspace_sizes(m, n, o)
+    if form == "fp32":  # FP32 products stage nothing
+        assert lh == 0
+        return
+    assert lh > 0
+    hw = later.hworkspace(m, n, o)
+    out = []
+    for h in (hw, hw[:-1], None):
+        Ad = _dev(later, A)
+        R, info = later.rgsqrf(Ad, o=o, info=True, hwork=h)
+        _sync()
+        assert info.status == 0
+        out.append((_np(Ad), _np(R), info.kernels))
+    for q in out[1:]:
+        assert np.array_equal(out[0][0], q[0]) and np.array_equal(out[0][1], q[1])
+    assert out[0][2] > out[1][2] == out[2][2]  # the split launches ran with the full arena only
+
+
 def test_rgsqrf_honours_lda(later, oracle):
     import torch
     m, n, ld = 1000, 200, 1037
