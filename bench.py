@@ -128,6 +128,148 @@ def vendor_sgeqrf(torch, A0, reps=2):
     return best
 
 
+def hbm_rate(m: int, n: int, ms: float) -> float:
+    """Algorithmic GB/s of a tall-skinny factorisation: read A, write Q (8 m n
+    bytes) and write R (4 n^2)."""
+    return (8.0 * m * n + 4.0 * n * n) / (ms * 1e-3) / 1e9
+
+
+def single_config(later, torch, name, m, n, kind, forms, steps=3):
+    """One factorisation configuration, timed by graph replay, per form."""
+    A0 = (later.generate_normal if kind == "normal" else later.generate_uniform)(m, n, seed=0, mat=0)
+    out = {"m": m, "n": n, "input": kind + " seed 0"}
+    for form in forms:
+        times, eager_ms, Q, R, _, _ = time_form(later, torch, A0, form, steps, 1)
+        ms = sum(times) / len(times)
+        out[form] = {"ms": ms, "tflops_ref": later.flops_ref(m, n) / ms / 1e9,
+                     "gbps": hbm_rate(m, n, ms), "orth": later.check_orth(Q),
+                     "backward": later.check_backward(A0, Q, R)}
+        log(f"[bench] {name} {m}x{n} {form}: {ms:.3f} ms  {out[form]['tflops_ref']:.2f} TFLOP/s(ref)  "
+            f"{out[form]['gbps']:.0f} GB/s  orth={out[form]['orth']:.2e} bwd={out[form]['backward']:.2e}")
+        del Q, R
+    del A0
+    torch.cuda.empty_cache()
+    return out
+
+
+def batched_config(later, torch, count=4096, form="mixed", steps=3):
+    """4096 independent matrices, m log-uniform in [64, 4096], n uniform in
+    [16, min(m, 256)], entries 2u - 1 (Philox, seed 0, matrix index i), all
+    factored by one graph replay of a batched plan."""
+    import numpy as np
+    rng = np.random.default_rng(0)
+    m = np.rint(np.exp(rng.uniform(np.log(64), np.log(4096), count))).astype(int)
+    n = np.array([rng.integers(16, min(mi, 256) + 1) for mi in m])
+    As = [later.generate_uniform(int(mi), int(ni), seed=0, mat=i) for i, (mi, ni) in enumerate(zip(m, n))]
+    A0 = [a.clone() for a in As]
+    Rs = [later.colmajor(int(ni), int(ni)) for ni in n]
+    t0 = time.time()
+    plan = later.BatchPlan(As, Rs, later.opts(form))
+    build_s = time.time() - t0
+    s = torch.cuda.current_stream()
+    times = []
+    for it in range(steps + 1):
+        for a, a0 in zip(As, A0):
+            a.copy_(a0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(s)
+        plan.execute()
+        e1.record(s)
+        torch.cuda.synchronize()
+        if it:
+            times.append(e0.elapsed_time(e1))
+    ms = sum(times) / len(times)
+    flops = float(np.sum(2.0 * m * n * n))
+    nbytes = float(np.sum(8.0 * m * n))
+    worst = 0.0
+    for i in range(0, count, 512):  # spot checks
+        worst = max(worst, later.check_orth(As[i]))
+    out = {"count": count, "form": form, "ms": ms, "tflops_mgs": flops / ms / 1e9, "gbps": nbytes / ms / 1e6,
+           "launches": plan.launches, "plan_build_s": build_s, "orth_max_sampled": worst,
+           "shapes": "m log-uniform [64, 4096], n uniform [16, min(m, 256)], seed 0"}
+    log(f"[bench] C4 batched {count} matrices ({form}): {ms:.2f} ms  {out['tflops_mgs']:.2f} TFLOP/s(2mn^2)  "
+        f"{out['gbps']:.0f} GB/s  launches={plan.launches}  build {build_s:.1f} s  orth<={worst:.1e}")
+    plan.close()
+    del As, A0, Rs
+    torch.cuda.empty_cache()
+    return out
+
+
+def serving_config(later, torch, count=64, rate=100.0, slots=8, budget=16 << 30):
+    """Least-squares serving: requests with m log-uniform in [128, 32768],
+    n log-uniform in [16, 1024] (n <= m), condition number log-uniform in
+    [1, 1e6] (A = U diag(s) V^T, s_i = cond^(-i/(n-1)), U and V from FP64 QR
+    of Gaussians), k uniform in [1, 16] right-hand sides, b = A x + 1e-3 noise,
+    arriving as a Poisson process of `rate` requests/s.  Latency is the
+    server's own submit-to-done time; x is checked against FP64 least squares."""
+    import numpy as np
+    rng = np.random.default_rng(1)
+    g = torch.Generator(device="cuda").manual_seed(1)
+    probs = []
+    for i in range(count):
+        m = int(round(np.exp(rng.uniform(np.log(128), np.log(32768)))))
+        n = min(m, int(round(np.exp(rng.uniform(np.log(16), np.log(1024))))))
+        k = int(rng.integers(1, 17))
+        cond = float(10 ** rng.uniform(0, 6))
+        U = torch.linalg.qr(torch.randn(m, n, device="cuda", dtype=torch.float64, generator=g))[0]
+        V = torch.linalg.qr(torch.randn(n, n, device="cuda", dtype=torch.float64, generator=g))[0]
+        sv = cond ** (-torch.arange(n, device="cuda", dtype=torch.float64) / max(n - 1, 1))
+        A64 = (U * sv) @ V.T
+        x = torch.randn(n, k, device="cuda", dtype=torch.float64, generator=g)
+        b64 = A64 @ x + 1e-3 * torch.randn(m, k, device="cuda", dtype=torch.float64, generator=g)
+        A = A64.float().t().contiguous().t()
+        B = b64.float().t().contiguous().t()
+        x64 = torch.linalg.lstsq(A.double(), B.double()).solution
+        probs.append((A, B, x64, m, n, k, cond))
+        del U, V, A64, b64
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    torch.cuda.synchronize()
+    srv = later.Server(slots=slots, hbm_budget_bytes=budget)
+    ids, lat, errs, done = [None] * count, [], [], 0
+    t0 = time.perf_counter()
+    nxt = 0
+    pending = set()
+    while done < count:
+        now = time.perf_counter() - t0
+        while nxt < count and arrivals[nxt] <= now:
+            A, B = probs[nxt][0], probs[nxt][1]
+            ids[nxt] = srv.submit(A, B)
+            pending.add(nxt)
+            nxt += 1
+        srv.poll()
+        for i in sorted(pending):
+            r = srv.result(ids[i])
+            if r["state"] in (3, 4):
+                A, B, x64, m, n, k, cond = probs[i]
+                if r["state"] == 3:
+                    X, _ = srv.fetch(ids[i], m, n, k)
+                    errs.append(float(torch.linalg.norm(X.double() - x64) / torch.linalg.norm(x64)))
+                    lat.append(r["latency_s"])
+                srv.release(ids[i])
+                pending.discard(i)
+                done += 1
+        if time.perf_counter() - t0 > 120:
+            break
+    wall = time.perf_counter() - t0
+    stats = srv.stats()
+    srv.close()
+    lat_ms = np.array(lat) * 1e3
+    out = {"requests": count, "completed": len(lat), "rate_per_s": rate, "slots": slots,
+           "p50_ms": float(np.percentile(lat_ms, 50)) if len(lat) else None,
+           "p99_ms": float(np.percentile(lat_ms, 99)) if len(lat) else None,
+           "solves_per_s": len(lat) / wall, "x_rel_err_median": float(np.median(errs)) if errs else None,
+           "x_rel_err_max": float(np.max(errs)) if errs else None, "peak_pool_bytes": stats["peak_bytes"],
+           "workload": "m log-uniform [128, 32768], n log-uniform [16, 1024], cond log-uniform [1, 1e6], "
+                       "k in [1, 16], Poisson arrivals"}
+    log(f"[bench] C5 serving {count} requests at {rate}/s: p50 {out['p50_ms']:.1f} ms  p99 {out['p99_ms']:.1f} ms  "
+        f"{out['solves_per_s']:.1f} solves/s  |x-x64|/|x64| median {out['x_rel_err_median']:.1e} "
+        f"max {out['x_rel_err_max']:.1e}")
+    del probs
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(n_sample: int):
     from oracle import oracle as O
     import numpy as np
@@ -154,6 +296,8 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSOLVER sgeqrf baseline")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the secondary workloads (8192x1024, tall-skinny, batched, serving)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this QR path runs on one GPU; multi-GPU row sharding is described "
@@ -203,6 +347,16 @@ def main():
     if vend:
         log(f"[bench] rocSOLVER sgeqrf: {vend:.2f} ms  {later.flops_ref(m, n) / vend / 1e9:.1f} TFLOP/s(ref)")
 
+    extra = None
+    if not args.no_extra:
+        extra = {
+            "C1": single_config(later, torch, "C1", 8192, 1024, "uniform", ("reference", "mixed")),
+            "C3a": single_config(later, torch, "C3a", 1048576, 128, "normal", ("reference", "mixed")),
+            "C3b": single_config(later, torch, "C3b", 262144, 256, "normal", ("reference", "mixed")),
+            "C4": batched_config(later, torch),
+            "C5": serving_config(later, torch),
+        }
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline sample n={args.cpu_sample}")
@@ -239,6 +393,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernel_classes": {k: v for k, v in pr.items()},
+        "other_workloads": extra,
     }
     print(json.dumps(out))
 

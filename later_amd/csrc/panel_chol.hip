@@ -16,7 +16,9 @@
 // inverse adds errors of order u_64 kappa, far below the FP32 rounding of Q.
 // Workgroups never wait for each other: the kernel boundaries order the
 // partials, so there are no atomics and no device-scope fences (which would
-// write back and invalidate the L2 of every XCD).
+// write back and invalidate the L2 of every XCD).  Tall panels (more than 32
+// workgroups, up to 256) add their partials in a small reduction kernel
+// before K2 and K3 instead of in every workgroup.
 //
 // One pass of Cholesky QR loses orthogonality like kappa^2 u; the second pass
 // starts from kappa(Q1) ~ 1 and restores O(u) for kappa(panel) up to ~1e7.
@@ -32,7 +34,8 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 constexpr int kW = 32;                    // max panel width
 constexpr int kRows = 256;                // rows per block = threads per workgroup
-constexpr int kMaxGroups = 32;            // workgroups (and Gram partials) per launch
+constexpr int kMaxGroups = 256;           // workgroups (and Gram partials) per launch
+constexpr int kDirect = 32;               // up to this many partials, every workgroup adds them itself
 constexpr int kPacked = kW * (kW + 1) / 2;  // packed upper triangle of a 32 x 32 Gram
 constexpr double kPivotRel = 64.0 * 2.220446049250313e-16;
 
@@ -40,6 +43,7 @@ struct CholWork {
     double *p1;  // kMaxGroups x kPacked: partial Grams of A
     double *p2;  // kMaxGroups x kPacked: partial Grams of Q1
     double *r1;  // kW x kW, row-major: R1 (upper)
+    double *g;   // kPacked: a reduced Gram (more than kDirect partials)
 };
 
 __host__ __device__ inline CholWork carve(float *ws)
@@ -48,6 +52,7 @@ __host__ __device__ inline CholWork carve(float *ws)
     c.p1 = (double *)ws;
     c.p2 = c.p1 + (int64_t)kMaxGroups * kPacked;
     c.r1 = c.p2 + (int64_t)kMaxGroups * kPacked;
+    c.g = c.r1 + kW * kW;
     return c;
 }
 
@@ -257,6 +262,36 @@ __device__ __forceinline__ void gram_lds(Gram3 &g, const float (*Qs)[kRows + 4],
     gram_step(g, a0, a1);
 }
 
+// Sum of more than kDirect partial Grams, in a fixed tree: workgroup w owns
+// entries 32w .. 32w+31; thread (q, e) adds groups q*G/8 .. (q+1)*G/8 - 1 in
+// order, then the eight part sums are added in order of q.
+__global__ __launch_bounds__(256) void chol_reduce_kernel(const double *part, int ngroups, double *out)
+{
+    __shared__ double ps[8][32];
+    const int e = blockIdx.x * 32 + (threadIdx.x & 31), q = threadIdx.x >> 5;
+    const int per = (ngroups + 7) / 8, g0 = q * per, g1 = min(ngroups, g0 + per);
+    double s = 0.0;
+    if (e < kPacked) {
+        constexpr int kB = 16;
+        for (int g = g0; g < g1; g += kB) {
+            double v[kB];
+#pragma unroll
+            for (int u = 0; u < kB; ++u) v[u] = part[(int64_t)min(g + u, g1 - 1) * kPacked + e];
+#pragma unroll
+            for (int u = 0; u < kB; ++u)
+                if (g + u < g1) s += v[u];
+        }
+    }
+    ps[q][threadIdx.x & 31] = s;
+    __syncthreads();
+    if (threadIdx.x < 32 && e < kPacked) {
+        double t = ps[0][threadIdx.x];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) t += ps[k][threadIdx.x];
+        out[e] = t;
+    }
+}
+
 __global__ __launch_bounds__(kRows) void chol_gram_kernel(int m, int w, const float *A, int64_t lda, float *ws)
 {
     __shared__ double Cb[4 * 3 * 4 * 64];
@@ -310,7 +345,8 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
     float v[kW];
     load_row(A, lda, m, w, blockIdx.x * kRows + t, v);
 
-    sum_partials(PASS == 1 ? cw.p1 : cw.p2, G, w, Gs);
+    if (G > kDirect) sum_partials(cw.g, 1, w, Gs);  // reduced by chol_reduce_kernel
+    else sum_partials(PASS == 1 ? cw.p1 : cw.p2, G, w, Gs);
     __syncthreads();
     if (t < 64) {
         const bool bad = cholesky_wave(Gs, Rs, Ris);
@@ -369,7 +405,7 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
 
 } // namespace
 
-int64_t panel_chol_work_floats() { return ((int64_t)2 * kMaxGroups * kPacked + kW * kW) * 2 + 64; }
+int64_t panel_chol_work_floats() { return ((int64_t)2 * kMaxGroups * kPacked + kW * kW + kPacked) * 2 + 64; }
 
 int panel_cholqr2(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr, float *work, int64_t lwork,
                   int *status, int *launches)
@@ -378,12 +414,16 @@ int panel_cholqr2(hipStream_t s, int m, int w, float *A, int lda, float *R, int 
     if (lwork < panel_chol_work_floats()) return LATER_ERR_WORKSPACE;
     const int nblk = (m + kRows - 1) / kRows;
     const int G = nblk < kMaxGroups ? nblk : kMaxGroups;
+    const CholWork cw = carve(work);
+    const dim3 rgrid((kPacked + 31) / 32);
     hipLaunchKernelGGL(chol_gram_kernel, dim3(G), dim3(kRows), 0, s, m, w, (const float *)A, (int64_t)lda, work);
+    if (G > kDirect) hipLaunchKernelGGL(chol_reduce_kernel, rgrid, dim3(256), 0, s, cw.p1, G, cw.g);
     hipLaunchKernelGGL(chol_apply_kernel<1>, dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
                        (int64_t)ldr, status);
+    if (G > kDirect) hipLaunchKernelGGL(chol_reduce_kernel, rgrid, dim3(256), 0, s, cw.p2, G, cw.g);
     hipLaunchKernelGGL(chol_apply_kernel<2>, dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
                        (int64_t)ldr, status);
     HIP_OK(hipGetLastError());
-    if (launches) *launches += 3;
+    if (launches) *launches += G > kDirect ? 5 : 3;
     return LATER_OK;
 }
