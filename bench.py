@@ -270,6 +270,20 @@ def serving_config(later, torch, count=64, rate=100.0, slots=8, budget=16 << 30)
     return out
 
 
+def read_traffic(k):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated on a device
+    copy of known size); None when the summary is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "gemm_update_x3_16384_traffic.txt")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    mt = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
+    return float(mt.group(1)) if mt else None
+
+
 def cpu_baseline(n_sample: int):
     from oracle import oracle as O
     import numpy as np
@@ -298,6 +312,10 @@ def main():
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSOLVER sgeqrf baseline")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary workloads (8192x1024, tall-skinny, batched, serving)")
+    ap.add_argument("--single", choices=("mixed", "fp32", "reference"),
+                    help="one eager factorisation of the headline workload and nothing else (for profilers)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="three 1 GiB device-to-device copies (the HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this QR path runs on one GPU; multi-GPU row sharding is described "
@@ -309,6 +327,23 @@ def main():
 
     n = args.n
     m = args.m or n
+    if args.copy_calibration:
+        x = torch.empty(1 << 28, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        x.fill_(1.0)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("[bench] copied 3 x 1 GiB")
+        return
+    if args.single:
+        o = later.opts(args.single)
+        A = later.generate_normal(m, n, seed=0, mat=0)
+        R = later.colmajor(n, n)
+        later.rgsqrf(A, R=R, o=o, work=later.workspace(m, n, o), hwork=later.hworkspace(m, n, o))
+        torch.cuda.synchronize()
+        log(f"[bench] one {args.single} factorisation of {m}x{n} done")
+        return
     log(f"[bench] device {torch.cuda.get_device_name(0)}  m={m} n={n}  steps={args.steps} warmup={args.warmup}")
     A0 = later.generate_normal(m, n, seed=0, mat=0)
     torch.cuda.synchronize()
@@ -332,16 +367,21 @@ def main():
             f"({time.time() - t0:.1f} s)")
 
     mixed, fp32 = res["mixed"], res["fp32"]
-    # Dominant kernel of the mixed form: the node GEMMs (projection + update).
+    # Dominant kernel of the mixed form: the FP16 hi/lo node update on
+    # pre-split operands (gemm_kernel<3, 128, 128, true, false, true>), timed
+    # per launch with HIP events on its stream.  achieved = its algorithmic
+    # flops 2 m n1 n2 per launch, summed, over its summed duration; each such
+    # flop costs three MFMA flops (hi*hi, hi*lo, lo*hi).
     pr = mixed["prof"]
-    gemm_ms = pr["gemm_proj"]["ms"] + pr["gemm_update"]["ms"]
-    gemm_fl = pr["gemm_proj"]["flops"] + pr["gemm_update"]["flops"]
-    launches = pr["gemm_proj"]["launches"] + pr["gemm_update"]["launches"]
-    achieved = gemm_fl / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    k = pr["gemm_update_x3"]
+    achieved = k["flops"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
+    traffic = read_traffic(k)
     roofline = {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK_TFLOPS["f16x3"], "unit": "TFLOP/s",
-                "frac": achieved / MFMA_PEAK_TFLOPS["f16x3"], "traffic": None,
-                "kernel": "gemm_kernel (node projection + update, FP16 hi/lo MFMA)",
-                "launches": launches, "kernel_ms": gemm_ms}
+                "frac": achieved / MFMA_PEAK_TFLOPS["f16x3"], "traffic": traffic,
+                "kernel": "gemm_kernel<3,128,128,true,false,true> (node update A2 -= Q1 R12, FP16 hi/lo pieces)",
+                "launches": k["launches"], "kernel_ms": k["ms"], "avg_launch_us": 1e3 * k["ms"] / max(k["launches"], 1),
+                "algorithmic_bytes_per_launch": k["bytes"] / max(k["launches"], 1),
+                "mfma_flops_issued_tflops": 3.0 * achieved}
 
     vend = None if args.no_vendor else vendor_sgeqrf(torch, A0)
     if vend:

@@ -491,7 +491,7 @@ class Server:
             self.h = None
 
 
-PROF_CLASSES = ("gemm_proj", "gemm_update", "panel_mgs", "other")
+PROF_CLASSES = ("gemm_proj", "gemm_update", "panel_mgs", "other", "gemm_update_x3")
 
 
 def prof_begin():

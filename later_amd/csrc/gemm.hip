@@ -866,7 +866,8 @@ int gemm_nn(hipStream_t s, int prec, int M, int N, int K, float alpha, const flo
     const bool small = (N <= 64);
     const int tok = prof_start(s);
     int rc = dispatch<false, false>(s, prec, g, 1, small);
-    prof_stop(tok, s, PROF_GEMM_UPDATE, tri ? (double)M * N * K / 3.0 : 2.0 * M * N * K,
+    const bool x3 = prec == GEMM_F16X3 && !small && g.Ah && g.Bh;
+    prof_stop(tok, s, x3 ? PROF_UPDATE_X3 : PROF_GEMM_UPDATE, tri ? (double)M * N * K / 3.0 : 2.0 * M * N * K,
               4.0 * ((double)M * K + 2.0 * M * N + (double)K * N));
     if (rc) return rc;
     if (launches) ++*launches;

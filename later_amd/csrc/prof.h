@@ -4,7 +4,11 @@
 #pragma once
 #include "common.h"
 
-enum { PROF_GEMM_PROJ = 0, PROF_GEMM_UPDATE = 1, PROF_PANEL = 2, PROF_OTHER = 3, PROF_NCLASS = 4 };
+// PROF_UPDATE_X3: the node updates of the FP16 hi/lo form on pre-split
+// operands with 128 x 128 tiles (one kernel, the mixed form's largest);
+// PROF_GEMM_UPDATE holds every other update.
+enum { PROF_GEMM_PROJ = 0, PROF_GEMM_UPDATE = 1, PROF_PANEL = 2, PROF_OTHER = 3, PROF_UPDATE_X3 = 4,
+       PROF_NCLASS = 5 };
 
 // Returns a token for prof_stop, or -1 when profiling is off.
 int prof_start(hipStream_t s);
