@@ -41,7 +41,7 @@ template <int PREC, bool CH64, int BM> constexpr int stages()
     if (PREC == GEMM_F32) return 2;
     if (BM == 64) return 4;
     if (PREC == GEMM_F16X3 && CH64) return 1;  // the FP64 fold's registers leave no room for more
-    return bk<PREC, CH64>() == 64 ? 2 : 3;
+    return 3;
 }
 
 template <int PREC> struct Lds;
@@ -300,6 +300,42 @@ __device__ __forceinline__ typename Frag<PREC>::T frag(const typename Lds<PREC>:
     }
 }
 
+// Epilogue of one wave's FM x FN accumulator blocks (value(i, j, r), FP64,
+// still scaled).  Transposed 32x32 layout: lane l holds row l%32 of the block
+// and columns 8 (r/4) + 4 (l/32) + r%4, r = 0..15.
+template <bool CH64, int FM, int FN, typename V>
+__device__ __forceinline__ void store_tile(const GemmArgs &g, int split, int row0, int col0, V value)
+{
+    const int lane = threadIdx.x & 63;
+    const float oscale_a = ldexpf(1.0f, -g.a_exp);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const int row = row0 + 32 * i + (lane & 31);
+        if (row >= g.M) continue;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int col = col0 + 32 * j + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (col >= g.N) continue;
+                const float cs = oscale_a * ((g.b_exp) ? ldexpf(1.0f, -g.b_exp[col]) : 1.0f);
+                const double v = value(i, j, r) * (double)cs;
+                if (g.partial) {
+                    const int64_t off = ((int64_t)split * g.N + col) * g.M + row;
+                    if constexpr (CH64) ((double *)g.partial)[off] = v;
+                    else ((float *)g.partial)[off] = (float)v;
+                } else if (g.update) {
+                    float *c = g.C + (int64_t)col * g.ldc + row;
+                    if (CH64 || g.update == 2) *c = (float)((double)*c + (double)g.alpha * v);
+                    else *c = fmaf(g.alpha, (float)v, *c);
+                } else {
+                    g.C[(int64_t)col * g.ldc + row] = (float)v;
+                }
+            }
+        }
+    }
+}
+
 // PRE: both operands come pre-split (GemmArgs.Ah / Bh, K-major 16-bit
 // pieces); otherwise they are read as FP32 and converted on the way to LDS.
 template <int PREC, int BM, int BN, bool AKM, bool CH64, bool PRE>
@@ -476,38 +512,10 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
         }
     }
 
-    // Epilogue.  Transposed 32x32 accumulator layout: lane l holds row l%32
-    // of the block and columns 8 (r/4) + 4 (l/32) + r%4, r = 0..15.
-    const float oscale_a = ldexpf(1.0f, -g.a_exp);
-#pragma unroll
-    for (int i = 0; i < FM; ++i) {
-        const int row = m0 + wm * WM + 32 * i + (lane & 31);
-        if (row >= g.M) continue;
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int col = n0 + wn * WN + 32 * j + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (col >= g.N) continue;
-                const float cs = oscale_a * ((g.b_exp) ? ldexpf(1.0f, -g.b_exp[col]) : 1.0f);
-                double v;
-                if constexpr (CH64) v = acc64[i][j][r];
-                else v = combined(i, j, r);
-                v *= (double)cs;
-                if (g.partial) {
-                    const int64_t off = ((int64_t)split * g.N + col) * g.M + row;
-                    if constexpr (CH64) ((double *)g.partial)[off] = v;
-                    else ((float *)g.partial)[off] = (float)v;
-                } else if (g.update) {
-                    float *c = g.C + (int64_t)col * g.ldc + row;
-                    if (CH64 || g.update == 2) *c = (float)((double)*c + (double)g.alpha * v);
-                    else *c = fmaf(g.alpha, (float)v, *c);
-                } else {
-                    g.C[(int64_t)col * g.ldc + row] = (float)v;
-                }
-            }
-        }
-    }
+    store_tile<CH64, FM, FN>(g, split, m0 + wm * WM, n0 + wn * WN, [&](int i, int j, int r) -> double {
+        if constexpr (CH64) return acc64[i][j][r];
+        else return combined(i, j, r);
+    });
 }
 
 // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs, each
@@ -515,8 +523,7 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
 // through a contiguous range, and each range walks the tiles in groups of 8
 // tile-rows (column by column inside a group), so the workgroups resident on
 // one XCD share their A and B tiles in its L2.
-template <int PREC, int BM, int BN, bool AKM, bool CH64, bool PRE>
-__global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
+__device__ __forceinline__ void tile_of(int &tm, int &tn, int &split)
 {
     constexpr int kXcd = 8, kGroup = 8;
     const int nx = gridDim.x, ny = gridDim.y;
@@ -525,12 +532,177 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
     const int q = total / kXcd;
     if (L < kXcd * q) L = (L % kXcd) * q + L / kXcd;
     const int tiles = nx * ny;
-    const int split = L / tiles;
+    split = L / tiles;
     const int r = L % tiles;
     const int first = (r / (kGroup * ny)) * kGroup;
     const int gsz = min(nx - first, kGroup);
     const int rr = r - first * ny;
-    gemm_body<PREC, BM, BN, AKM, CH64, PRE>(g, first + rr % gsz, rr / gsz, split);
+    tm = first + rr % gsz;
+    tn = rr / gsz;
+}
+
+template <int PREC, int BM, int BN, bool AKM, bool CH64, bool PRE>
+__global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs g)
+{
+    int tm, tn, split;
+    tile_of(tm, tn, split);
+    gemm_body<PREC, BM, BN, AKM, CH64, PRE>(g, tm, tn, split);
+}
+
+// s_waitcnt with vmcnt <= N and the other counters left alone (gfx9 layout:
+// vmcnt[3:0] and [15:14], expcnt[6:4], lgkmcnt[11:8]).
+template <int N> __device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// Pre-split operands streamed straight into LDS (global_load_lds_dwordx4: each
+// lane's 16 bytes land at the wave's LDS base + 16 * lane, no registers).
+// 128 x 128 tiles, four waves of 64 x 64, slabs of 32 k.  In a stage, each
+// piece of each operand tile is 128 rows of 64 bytes whose four 16-byte
+// chunks are XOR-swizzled by (row / 2) % 4; a fragment read (8 rows x one
+// chunk per 8 lanes) then touches 8 distinct bank groups.  NS stages are in
+// flight: a wave waits on its own loads with s_waitcnt, the barrier publishes
+// everyone's, and the freed stage is refilled right after it.  The piece
+// arrays must be zero-padded in K to a multiple of 32 (rows of A or B past M
+// or N are clamped: they only feed outputs that are not stored).
+template <int PREC, bool CH64>
+__device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, const int tn, const int split)
+{
+    constexpr int BK = 32, WM = 64, WN = 64, FM = 2, FN = 2, KR = 16, KSTEPS = 2;
+    constexpr int NP = pieces<PREC>(), NG = NP;
+    constexpr int NS = NP == 3 ? 3 : 4;      // stages: 48 or 32 KB each (X6 / X3), 16 KB for one piece
+    constexpr int PB = 128 * BK * 2;          // bytes of one piece of one operand tile
+    constexpr int SB = 2 * NP * PB;           // bytes of one stage
+    constexpr int LPS = 2 * NP * 2;           // loads per wave per stage
+    typedef typename Frag<PREC>::T FT;
+    __shared__ __attribute__((aligned(16))) char lds[NS * SB];
+
+    const int m0 = tm * 128, n0 = tn * 128;
+    const int kbeg = split * g.kper;
+    const int kend = min(g.K, kbeg + g.kper);
+    const int ntiles = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    auto issue = [&](int slab, int stage) {
+        const int k0 = kbeg + slab * BK;
+        char *sbase = lds + stage * SB;
+#pragma unroll
+        for (int op = 0; op < 2; ++op) {
+            const uint16_t *P = op ? g.Bh : g.Ah;
+            const int64_t ld = op ? g.ldbh : g.ldah, ps = op ? g.psb : g.psa;
+            const int r0 = op ? n0 : m0, rmax = op ? g.N : g.M;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int blk = 2 * wave + i;  // 1 KB block of the piece tile
+                const int q = blk * 64 + lane;
+                const int r = q >> 2;
+                const int cg = (q & 3) ^ ((r >> 1) & 3);
+                const uint16_t *src = P + (int64_t)min(r0 + r, rmax - 1) * ld + k0 + 8 * cg;
+#pragma unroll
+                for (int c = 0; c < NP; ++c)
+                    __builtin_amdgcn_global_load_lds(
+                        (const void *)(src + c * ps),
+                        (__attribute__((address_space(3))) void *)(sbase + (op * NP + c) * PB + blk * 1024), 16, 0,
+                        0);
+            }
+        }
+    };
+    auto frag = [&](const char *pbase, int row, int ks) -> FT {
+        const int ch = (2 * ks + (lane >> 5)) ^ ((row >> 1) & 3);
+        return *(const FT *)(pbase + row * 64 + ch * 16);
+    };
+
+    floatx16 acc[NG][FM][FN];
+    double acc64[CH64 ? FM : 1][CH64 ? FN : 1][CH64 ? 16 : 1];
+#pragma unroll
+    for (int q = 0; q < NG; ++q)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[q][i][j][r] = 0.0f;
+    if constexpr (CH64) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc64[i][j][r] = 0.0;
+    }
+    auto combined = [&](int i, int j, int r) -> double {
+        double v = (double)acc[0][i][j][r];
+        if constexpr (NG > 1) v += (double)acc[1][i][j][r] * 0x1p-11;
+        if constexpr (NG > 2) v += (double)acc[2][i][j][r] * 0x1p-22;
+        return v;
+    };
+    auto fold = [&]() {
+        if constexpr (CH64) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        acc64[i][j][r] += combined(i, j, r);
+#pragma unroll
+                        for (int q = 0; q < NG; ++q) acc[q][i][j][r] = 0.0f;
+                    }
+        }
+    };
+
+#pragma unroll
+    for (int st = 0; st < NS - 1; ++st)
+        if (st < ntiles) issue(st, st);
+    for (int it = 0; it < ntiles; ++it) {
+        if (it + NS - 2 < ntiles) wait_vmcnt<(NS - 2) * LPS>();
+        else wait_vmcnt<0>();
+        // A bare barrier: this wave's loads of stage `it` are complete (the
+        // wait above), so after it every wave's are; __syncthreads() would
+        // also wait for the later stages still in flight.  The memory clobber
+        // keeps LDS reads and the refill below on their side of it.
+        asm volatile("s_barrier" ::: "memory");
+        if (it + NS - 1 < ntiles) issue(it + NS - 1, (it + NS - 1) % NS);
+        const char *sb = lds + (it % NS) * SB;
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            FT fa[NP][FM], fb[NP][FN];
+#pragma unroll
+            for (int c = 0; c < NP; ++c) {
+#pragma unroll
+                for (int i = 0; i < FM; ++i) fa[c][i] = frag(sb + c * PB, wm * WM + 32 * i + (lane & 31), ks);
+#pragma unroll
+                for (int j = 0; j < FN; ++j) fb[c][j] = frag(sb + (NP + c) * PB, wn * WN + 32 * j + (lane & 31), ks);
+            }
+#pragma unroll
+            for (int gq = 0; gq < NG; ++gq)
+#pragma unroll
+                for (int a = 0; a <= gq; ++a)
+#pragma unroll
+                    for (int i = 0; i < FM; ++i)
+#pragma unroll
+                        for (int j = 0; j < FN; ++j)
+                            acc[gq][i][j] = mfma<PREC>(fb[gq - a][j], fa[a][i], acc[gq][i][j]);
+            if (CH64 && ks + 1 < KSTEPS && (it * BK + (ks + 1) * KR) % g.chunk_rows == 0) fold();
+        }
+        if (CH64 && (((it + 1) * BK) % g.chunk_rows == 0 || it + 1 == ntiles)) fold();
+    }
+
+    store_tile<CH64, FM, FN>(g, split, m0 + wm * WM, n0 + wn * WN, [&](int i, int j, int r) -> double {
+        if constexpr (CH64) return acc64[i][j][r];
+        else return combined(i, j, r);
+    });
+}
+
+template <int PREC, bool CH64>
+__global__ __launch_bounds__(NT) void gemm_lds_kernel(GemmArgs g)
+{
+    int tm, tn, split;
+    tile_of(tm, tn, split);
+    gemm_lds_body<PREC, CH64>(g, tm, tn, split);
 }
 
 // Grouped launch: workgroup b computes tile (t.y, t.z) of split t.w of the
@@ -602,6 +774,8 @@ __global__ void colexp_grouped_kernel(const ColexpArgs *args, const int2 *tasks)
 
 // Pieces of X scaled by 2^e (e = a_exp or col_exp[j]), same layout: each
 // thread converts 4 consecutive rows of one column.
+// Rows rows .. roundup(rows, 64) - 1 (the K padding the streaming GEMM
+// reads) are written as zeros.
 template <int PREC>
 __global__ void split_kernel(int rows, int cols, const float *X, int64_t ldx, int a_exp, const int *col_exp,
                              uint16_t *out, int64_t ldo, int64_t stride)
@@ -610,7 +784,8 @@ __global__ void split_kernel(int rows, int cols, const float *X, int64_t ldx, in
     constexpr int NP = pieces<PREC>();
     const int j = blockIdx.y;
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
-    if (i0 >= rows) return;
+    const int prow = (rows + 63) / 64 * 64;
+    if (i0 >= prow) return;
     const float sc = ldexpf(1.0f, col_exp ? col_exp[j] : a_exp);
     const float *x = X + (int64_t)j * ldx;
     uint16_t *o = out + (int64_t)j * ldo;
@@ -622,6 +797,9 @@ __global__ void split_kernel(int rows, int cols, const float *X, int64_t ldx, in
             split_value<PREC>(x[i] * sc, pc);
 #pragma unroll
             for (int c = 0; c < NP; ++c) o[c * stride + i] = __builtin_bit_cast(uint16_t, pc[c]);
+        } else if (i < prow) {
+#pragma unroll
+            for (int c = 0; c < NP; ++c) o[c * stride + i] = 0;
         }
     }
 }
@@ -649,7 +827,7 @@ __global__ __launch_bounds__(256) void split_t_kernel(int rows, int cols, const 
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int ii = ty + 4 * q, i = i0 + ii, j = j0 + tx;
-        if (i < rows && j < cols) {
+        if (i < rows) {  // columns cols .. roundup(cols, 64) - 1: the zero K padding
             T pc[NP];
             split_value<PREC>(tile[tx][ii], pc);
 #pragma unroll
@@ -667,7 +845,7 @@ void launch_split(hipStream_t s, int rows, int cols, const float *X, int64_t ldx
         hipLaunchKernelGGL(split_t_kernel<PREC>, grid, dim3(256), 0, s, rows, cols, X, ldx, a_exp, col_exp, out, ldo,
                            stride);
     } else {
-        dim3 grid((unsigned)ceil_div64(rows, 1024), (unsigned)cols);
+        dim3 grid((unsigned)ceil_div64((rows + 63) / 64 * 64, 1024), (unsigned)cols);
         hipLaunchKernelGGL(split_kernel<PREC>, grid, dim3(256), 0, s, rows, cols, X, ldx, a_exp, col_exp, out, ldo,
                            stride);
     }
@@ -683,6 +861,13 @@ template <int PREC, bool AKM, bool CH64, bool PRE>
 void launch_tile(hipStream_t s, const GemmArgs &g, int nsplit, bool small)
 {
     const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
+    if constexpr (PRE) {
+        if (!small) {
+            const dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
+            hipLaunchKernelGGL((gemm_lds_kernel<PREC, CH64>), grid, dim3(NT), 0, s, g);
+            return;
+        }
+    }
     if constexpr (PREC == GEMM_F16X6) {
         // Three accumulator groups: a 128x128 tile would spill, so the
         // three-piece form always runs 64x64 tiles.
