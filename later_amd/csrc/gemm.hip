@@ -567,21 +567,31 @@ template <int N> __device__ __forceinline__ void wait_vmcnt()
 // everyone's, and the freed stage is refilled right after it.  The piece
 // arrays must be zero-padded in K to a multiple of 32 (rows of A or B past M
 // or N are clamped: they only feed outputs that are not stored).
-template <int PREC, bool CH64>
+template <int PREC, bool CH64, int NW>
 __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, const int tn, const int split)
 {
-    constexpr int BK = 32, WM = 64, WN = 64, FM = 2, FN = 2, KR = 16, KSTEPS = 2;
+    // NW = 4: a 2 x 2 grid of 64 x 64 wave tiles; NW = 8: 4 x 2 of 32 x 64
+    // (half the accumulators per wave, for the three-piece form).
+    constexpr int BK = 32, WGM = NW / 2, WM = 128 / WGM, WN = 64, FM = WM / 32, FN = 2, KR = 16, KSTEPS = 2;
+    constexpr int BPW = 8 / NW;               // 1 KB blocks per wave per piece tile
     constexpr int NP = pieces<PREC>(), NG = NP;
     constexpr int NS = NP == 3 ? 3 : 4;      // stages: 48 or 32 KB each (X6 / X3), 16 KB for one piece
     constexpr int PB = 128 * BK * 2;          // bytes of one piece of one operand tile
     constexpr int SB = 2 * NP * PB;           // bytes of one stage
-    constexpr int LPS = 2 * NP * 2;           // loads per wave per stage
+    constexpr int LPS = 2 * NP * BPW;         // loads per wave per stage
     typedef typename Frag<PREC>::T FT;
     __shared__ __attribute__((aligned(16))) char lds[NS * SB];
 
     const int m0 = tm * 128, n0 = tn * 128;
-    const int kbeg = split * g.kper;
-    const int kend = min(g.K, kbeg + g.kper);
+    int kbeg = split * g.kper;
+    int kend = min(g.K, kbeg + g.kper);
+    if (g.tri) {
+        // Upper-triangular operands: tiles below the diagonal get nothing and
+        // only k in [m0, n0 + 128) contributes (both multiples of the slab).
+        if (m0 > n0 + 127) return;
+        kbeg = max(kbeg, m0);
+        kend = min(kend, n0 + 128);
+    }
     const int ntiles = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = wave >> 1, wn = wave & 1;
@@ -595,8 +605,8 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
             const int64_t ld = op ? g.ldbh : g.ldah, ps = op ? g.psb : g.psa;
             const int r0 = op ? n0 : m0, rmax = op ? g.N : g.M;
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int blk = 2 * wave + i;  // 1 KB block of the piece tile
+            for (int i = 0; i < BPW; ++i) {
+                const int blk = BPW * wave + i;  // 1 KB block of the piece tile
                 const int q = blk * 64 + lane;
                 const int r = q >> 2;
                 const int cg = (q & 3) ^ ((r >> 1) & 3);
@@ -697,12 +707,12 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
     });
 }
 
-template <int PREC, bool CH64>
-__global__ __launch_bounds__(NT) void gemm_lds_kernel(GemmArgs g)
+template <int PREC, bool CH64, int NW>
+__global__ __launch_bounds__(64 * NW) void gemm_lds_kernel(GemmArgs g)
 {
     int tm, tn, split;
     tile_of(tm, tn, split);
-    gemm_lds_body<PREC, CH64>(g, tm, tn, split);
+    gemm_lds_body<PREC, CH64, NW>(g, tm, tn, split);
 }
 
 // Grouped launch: workgroup b computes tile (t.y, t.z) of split t.w of the
@@ -731,15 +741,28 @@ __global__ void splitk_reduce_kernel(int M, int N, int nsplit, const void *parti
 }
 
 // Per-column exponents e_j with max_i |X_ij| 2^e_j in [2^14, 2^15) (0 for a
-// zero column).  One wave per column.
-__global__ void colexp_kernel(int m, int n, const float *X, int64_t ldx, int *e)
+// zero column).  One 256-thread workgroup per column, 16-byte loads where the
+// column is aligned (the maximum does not depend on the order).
+__global__ __launch_bounds__(256) void colexp_kernel(int m, int n, const float *X, int64_t ldx, int *e)
 {
+    __shared__ float wmax[4];
     const int j = blockIdx.x;
     const float *x = X + (int64_t)j * ldx;
     float mx = 0.0f;
-    for (int i = threadIdx.x; i < m; i += 64) mx = fmaxf(mx, fabsf(x[i]));
+    const int head = (int)min((int64_t)m, (int64_t)((16 - ((uintptr_t)x & 15)) & 15) / 4);
+    for (int i = threadIdx.x; i < head; i += 256) mx = fmaxf(mx, fabsf(x[i]));
+    const int nv = (m - head) / 4;
+    const floatx4 *xv = (const floatx4 *)(x + head);
+    for (int v = threadIdx.x; v < nv; v += 256) {
+        const floatx4 q = xv[v];
+        mx = fmaxf(mx, fmaxf(fmaxf(fabsf(q[0]), fabsf(q[1])), fmaxf(fabsf(q[2]), fabsf(q[3]))));
+    }
+    for (int i = head + 4 * nv + threadIdx.x; i < m; i += 256) mx = fmaxf(mx, fabsf(x[i]));
 #pragma unroll
     for (int mask = 32; mask > 0; mask >>= 1) mx = fmaxf(mx, __shfl_xor(mx, mask, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
     if (threadIdx.x == 0) {
         int ex = 0;
         if (mx > 0.0f && mx < INFINITY) {
@@ -863,8 +886,11 @@ void launch_tile(hipStream_t s, const GemmArgs &g, int nsplit, bool small)
     const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
     if constexpr (PRE) {
         if (!small) {
+            // Eight waves for the three-piece form, whose four-wave tiles
+            // would not fit their accumulators in registers.
+            constexpr int NW = PREC == GEMM_F16X6 ? 8 : 4;
             const dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
-            hipLaunchKernelGGL((gemm_lds_kernel<PREC, CH64>), grid, dim3(NT), 0, s, g);
+            hipLaunchKernelGGL((gemm_lds_kernel<PREC, CH64, NW>), grid, dim3(64 * NW), 0, s, g);
             return;
         }
     }
@@ -1062,7 +1088,7 @@ int gemm_nn(hipStream_t s, int prec, int M, int N, int K, float alpha, const flo
 int gemm_colexp(hipStream_t s, int m, int n, const float *X, int64_t ldx, int *e, int *launches)
 {
     if (n <= 0) return LATER_OK;
-    hipLaunchKernelGGL(colexp_kernel, dim3(n), dim3(64), 0, s, m, n, X, ldx, e);
+    hipLaunchKernelGGL(colexp_kernel, dim3(n), dim3(256), 0, s, m, n, X, ldx, e);
     HIP_OK(hipGetLastError());
     if (launches) ++*launches;
     return LATER_OK;

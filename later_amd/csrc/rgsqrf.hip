@@ -47,10 +47,8 @@ constexpr int kPieceMinN = 512;
 // channels.
 int64_t piece_ld(int64_t x) { return (x + 7) / 8 * 8 + 64; }
 
-// Pieces of the projection operands: not for the three-piece form, whose
-// 6 bytes per element cost more to stream than the in-kernel split (measured
-// at the 16384^2 top node).
-int proj_pieces(int prec) { return prec == LATER_GEMM_F16X6 ? 0 : gemm_pieces(prec); }
+// Pieces of the projection operands.
+int proj_pieces(int prec) { return gemm_pieces(prec); }
 
 // 16-bit elements the pre-split operands of one node need: Q1 (column-major
 // and transposed), A2 and R12.
@@ -95,6 +93,11 @@ Layout layout(int m, int n, const later_opts &o)
     if ((o.reorth & LATER_REORTH_PASS) && o.gemm_pass2 != LATER_GEMM_F32 && n > o.nmin) {
         const int64_t p2 = node_pieces(m, n, o.gemm_pass2);
         if (p2 > ph) ph = p2;
+    }
+    // The final R = R1 + (R2 - I) R1 stages R2 (transposed) and R1.
+    if ((o.reorth & LATER_REORTH_PASS) && o.gemm_pass2 != LATER_GEMM_F32 && n >= kPieceMinN) {
+        const int64_t pt = 2 * gemm_pieces(update_prec(o.gemm_pass2)) * piece_ld(n) * n;
+        if (pt > ph) ph = pt;
     }
     L.hpieces = ph;
     return L;
@@ -371,15 +374,26 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
             hipLaunchKernelGGL(copy_upper_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R2, (int64_t)n, R2,
                                1.0f);
             r.launches += 2;
-            const int p2 = o.gemm_pass2;
+            const int p2 = o.gemm_pass2, pu = update_prec(p2);
             const int *bexp = nullptr;
             if (scaled(p2)) {
                 rc = gemm_colexp(r.s, n, n, T, n, r.exps, &r.launches);
                 bexp = r.exps;
             }
+            const int aexp = scaled(p2) ? kQExp : 0;
+            const bool pre = r.pcs && n >= kPieceMinN && gemm_pieces(pu) > 0;
+            Pieces rt{}, tt{};
+            if (!rc && pre) {
+                const int64_t ld = piece_ld(n);
+                rt = {r.pcs, ld, ld * n};
+                tt = {r.pcs + gemm_pieces(pu) * ld * n, ld, ld * n};
+                rc = gemm_split(r.s, pu, n, n, R2, n, aexp, nullptr, (uint16_t *)rt.p, ld, ld * n, true, &r.launches);
+                if (!rc)
+                    rc = gemm_split(r.s, pu, n, n, T, n, 0, bexp, (uint16_t *)tt.p, ld, ld * n, false, &r.launches);
+            }
             if (!rc)
-                rc = gemm_nn(r.s, update_prec(p2), n, n, n, 1.0f, R2, n, T, n, R, ldr, o.chunk > 0, bexp,
-                             scaled(p2) ? kQExp : 0, true, &r.launches);
+                rc = gemm_nn(r.s, pu, n, n, n, 1.0f, R2, n, T, n, R, ldr, o.chunk > 0, bexp, aexp, true,
+                             &r.launches, pre ? &rt : nullptr, pre ? &tt : nullptr);
         }
     }
     if (!rc && hipGetLastError() != hipSuccess) rc = LATER_ERR_DEVICE;
