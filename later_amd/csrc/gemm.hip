@@ -725,17 +725,27 @@ __global__ __launch_bounds__(NT) void gemm_grouped_kernel(const GemmArgs *args, 
     gemm_body<PREC, BM, BN, AKM, CH64, false>(g, t.y, t.z, t.w);
 }
 
-// C = sum over splits (fixed order, FP64) of the partials.
+// C = sum over splits (fixed order, FP64) of the partials; 16 loads in
+// flight per thread (the sum itself stays sequential in p).
 template <bool F64>
 __global__ void splitk_reduce_kernel(int M, int N, int nsplit, const void *partial, float *C, int64_t ldc)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)M * N) return;
+    const int64_t tot = (int64_t)M * N;
+    if (idx >= tot) return;
     const int row = (int)(idx % M), col = (int)(idx / M);
+    constexpr int kB = 16;
     double s = 0.0;
-    for (int p = 0; p < nsplit; ++p) {
-        const int64_t off = (int64_t)p * M * N + idx;
-        s += F64 ? ((const double *)partial)[off] : (double)((const float *)partial)[off];
+    for (int p0 = 0; p0 < nsplit; p0 += kB) {
+        double v[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int64_t off = (int64_t)min(p0 + u, nsplit - 1) * tot + idx;
+            v[u] = F64 ? ((const double *)partial)[off] : (double)((const float *)partial)[off];
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u)
+            if (p0 + u < nsplit) s += v[u];
     }
     C[(int64_t)col * ldc + row] = (float)s;
 }

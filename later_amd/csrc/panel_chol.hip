@@ -205,46 +205,6 @@ __device__ __forceinline__ bool cholesky_wave(const double (*Gs)[kW + 1], double
     return bad;
 }
 
-// Loads row r (clamped) of the m x w panel as 32 floats (zero beyond w, m).
-__device__ __forceinline__ void load_row(const float *A, int64_t lda, int m, int w, int row, float v[kW])
-{
-    const int rc = min(row, m - 1);
-#pragma unroll
-    for (int k = 0; k < kW; ++k) {
-        const float x = A[rc + (int64_t)min(k, w - 1) * lda];
-        v[k] = (row < m && k < w) ? x : 0.0f;
-    }
-}
-
-// q = a R^{-1} for one row in FP64 (R^{-1} upper, in LDS).  Row c + 1 of
-// R^{-1} is read while row c is applied.  The scheduling barrier and the
-// empty asm statements keep every step's reads and updates in that step:
-// hoisted or sunk, all of R^{-1} would be live at once and would not fit in
-// registers.
-__device__ __forceinline__ void apply_row(const float v[kW], const double (*Ris)[kW], double q[kW])
-{
-    double cur[kW], nxt[kW];
-#pragma unroll
-    for (int jj = 0; jj < kW; ++jj) {
-        q[jj] = 0.0;
-        cur[jj] = Ris[0][jj];
-    }
-#pragma unroll
-    for (int c = 0; c < kW; ++c) {
-        __builtin_amdgcn_sched_barrier(0);
-        if (c + 1 < kW)
-#pragma unroll
-            for (int jj = c + 1; jj < kW; ++jj) nxt[jj] = Ris[c + 1][jj];
-        const double x = (double)v[c];
-#pragma unroll
-        for (int jj = c; jj < kW; ++jj) q[jj] = fma(x, cur[jj], q[jj]);
-#pragma unroll
-        for (int jj = c; jj < kW; ++jj) asm volatile("" : "+v"(q[jj]));
-#pragma unroll
-        for (int jj = c + 1; jj < kW; ++jj) cur[jj] = nxt[jj];
-    }
-}
-
 // Gram MFMA operands of one 16-row group at local row r0 + 4 (l/16) of a
 // column-major LDS block Qs[col][kRows + 4].
 __device__ __forceinline__ void gram_lds(Gram3 &g, const float (*Qs)[kRows + 4], int r0)
@@ -327,6 +287,35 @@ __global__ __launch_bounds__(kRows) void chol_gram_kernel(int m, int w, const fl
 //           stored; partials p2 of the stored Q1.
 //   PASS 2: partials p2 -> R2; Q = Q1 R2^{-1} stored; workgroup 0 writes
 //           R = R2 R1 (ldr, FP32).
+// Rows of one 256-row block in the operand layout of the apply MFMA: wave wv
+// owns rows 64 wv .. 64 wv + 63 as four 16-row tiles; lane l holds, for tile
+// t and k-step s, A[r0 + 64 wv + 16 t + l%16][4 s + l/16] (zero past m or w).
+__device__ __forceinline__ void load_block_mfma(const float *A, int64_t lda, int m, int w, int r0, float av[4][8])
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int kk = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int row = r0 + wv * 64 + t * 16 + (lane & 15);
+        const int rc = min(row, m - 1);
+#pragma unroll
+        for (int s2 = 0; s2 < 8; ++s2) {
+            const int col = 4 * s2 + kk;
+            const float x = A[rc + (int64_t)min(col, w - 1) * lda];
+            av[t][s2] = (row < m && col < w) ? x : 0.0f;
+        }
+    }
+}
+
+// One Cholesky QR pass over the rows of this workgroup.
+//   PASS 1: Gram of A -> R1 (stored to r1 by workgroup 0); Q1 = A R1^{-1}
+//           stored; partials p2 of the stored Q1.
+//   PASS 2: Gram of Q1 -> R2; Q = Q1 R2^{-1} stored; workgroup 0 writes
+//           R = R2 R1 (ldr, FP32).
+// The product with R^{-1} runs on FP64 MFMA as D = R^{-T} A^T per 16 x 16
+// tile: the A operand (slices of R^{-1}) stays in registers for the whole
+// block, the B operand is the block's rows, and D's lanes walk 16
+// consecutive rows of one column (64-byte stores).
 template <int PASS>
 __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *A, int64_t lda, float *ws, float *R,
                                                           int64_t ldr, int *status)
@@ -340,10 +329,11 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
     const int nblk = (m + kRows - 1) / kRows;
     const int G = gridDim.x;
     const int t = threadIdx.x;
+    const int lane = t & 63, wv = t >> 6;
 
     // The first block's rows are in flight while the factor is formed.
-    float v[kW];
-    load_row(A, lda, m, w, blockIdx.x * kRows + t, v);
+    float av[4][8];
+    load_block_mfma(A, lda, m, w, blockIdx.x * kRows, av);
 
     if (G > kDirect) sum_partials(cw.g, 1, w, Gs);  // reduced by chol_reduce_kernel
     else sum_partials(PASS == 1 ? cw.p1 : cw.p2, G, w, Gs);
@@ -370,37 +360,53 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
         }
     }
 
+    // A-operand slices of R^{-1}: rv[ct][s] = R^{-1}[4 s + l/16][16 ct + l%16]
+    // (zero for k > column, so column tile 0 needs k-steps 0..3 only).
+    double rv[2][8];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int s2 = 0; s2 < 8; ++s2) rv[ct][s2] = Ris[4 * s2 + (lane >> 4)][16 * ct + (lane & 15)];
+
     Gram3 g;
     g.t00 = g.t01 = g.t11 = doublex4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
     for (int b = blockIdx.x; b < nblk; b += G) {
-        // Also keeps the loop-invariant reads of R inside the loop: hoisted,
-        // 528 doubles would not fit in registers.
-        __syncthreads();
+        __syncthreads();  // Qs of the previous block has been read
         const int r0 = b * kRows;
-        const int rows = min(kRows, m - r0);
-        double q[kW];
-        apply_row(v, Ris, q);
-        float qf[kW];
 #pragma unroll
-        for (int jj = 0; jj < kW; ++jj) qf[jj] = (t < rows && jj < w) ? (float)q[jj] : 0.0f;
-        if (t < rows)
+        for (int tt = 0; tt < 4; ++tt) {
+            const int lrow = wv * 64 + tt * 16 + (lane & 15);  // row within the block
+            const int row = r0 + lrow;
 #pragma unroll
-            for (int jj = 0; jj < kW; ++jj)
-                if (jj < w) A[r0 + t + (int64_t)jj * lda] = qf[jj];
-        if (b + G < nblk) load_row(A, lda, m, w, (b + G) * kRows + t, v);
+            for (int ct = 0; ct < 2; ++ct) {
+                doublex4 d = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s2 = 0; s2 < (ct == 0 ? 4 : 8); ++s2)
+                    d = __builtin_amdgcn_mfma_f64_16x16x4f64(rv[ct][s2], (double)av[tt][s2], d, 0, 0, 0);
+                // d[r] = Q[row][16 ct + l/16 + 4 r]
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = 16 * ct + (lane >> 4) + 4 * r;
+                    const bool ok = row < m && col < w;
+                    const float q = ok ? (float)d[r] : 0.0f;
+                    if (ok) A[row + (int64_t)col * lda] = q;
+                    if constexpr (PASS == 1) Qs[col][lrow] = q;
+                }
+            }
+        }
+        if (b + G < nblk) load_block_mfma(A, lda, m, w, (b + G) * kRows, av);
         if constexpr (PASS == 1) {
             // Gram of the stored (FP32) Q1 rows of this block.
-#pragma unroll
-            for (int jj = 0; jj < kW; ++jj) Qs[jj][t] = qf[jj];
             __syncthreads();
-            const int wv = t >> 6;
 #pragma unroll
             for (int sg = 0; sg < 4; ++sg) gram_lds(g, Qs, wv * 64 + sg * 16);
-            __syncthreads();
         }
     }
-    if constexpr (PASS == 1) gram_finish(g, Cb, cw.p2 + (int64_t)blockIdx.x * kPacked);
+    if constexpr (PASS == 1) {
+        __syncthreads();
+        gram_finish(g, Cb, cw.p2 + (int64_t)blockIdx.x * kPacked);
+    }
 }
 
 } // namespace
