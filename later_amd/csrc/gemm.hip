@@ -41,7 +41,7 @@ template <int PREC, bool CH64, int BM> constexpr int stages()
     if (PREC == GEMM_F32) return 2;
     if (BM == 64) return 4;
     if (PREC == GEMM_F16X3 && CH64) return 1;  // the FP64 fold's registers leave no room for more
-    return 3;
+    return bk<PREC, CH64>() == 64 ? 2 : 3;      // 64-deep slabs of the hi/lo pair: 3 would spill
 }
 
 template <int PREC> struct Lds;
@@ -446,6 +446,8 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
 
     const int ntiles = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
 
+    const int per_ks = CH64 ? g.chunk_rows / KR : 1;  // k-steps per FP64 fold
+    int ksince = 0;
     // One slab of MFMAs from LDS buffer `cur` (slab `it` of this split).
     auto compute = [&](int cur, int it) {
 #pragma unroll
@@ -470,10 +472,15 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
 #pragma unroll
                         for (int j = 0; j < FN; ++j)
                             acc[gq][i][j] = mfma<PREC>(fb[gq - a][j], fa[a][i], acc[gq][i][j]);
-            // FP64 fold every chunk_rows rows of K (a multiple of 32).
-            if (CH64 && ks + 1 < KSTEPS && (it * BK + (ks + 1) * KR) % g.chunk_rows == 0) fold();
+            // FP64 fold every chunk_rows rows of K (a multiple of 32) and
+            // after the last k-step, counted rather than divided.
+            if constexpr (CH64) {
+                if (++ksince == per_ks || (ks + 1 == KSTEPS && it + 1 == ntiles)) {
+                    fold();
+                    ksince = 0;
+                }
+            }
         }
-        if (CH64 && (((it + 1) * BK) % g.chunk_rows == 0 || it + 1 == ntiles)) fold();
     };
     auto load = [&](int slot_it, SA &a, SB &b) {
         const int k0 = kbeg + slot_it * BK;
@@ -664,6 +671,8 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
         }
     };
 
+    const int per = CH64 ? g.chunk_rows / BK : 1;  // slabs per FP64 fold
+    int since = 0;
 #pragma unroll
     for (int st = 0; st < NS - 1; ++st)
         if (st < ntiles) issue(st, st);
@@ -696,9 +705,15 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
 #pragma unroll
                         for (int j = 0; j < FN; ++j)
                             acc[gq][i][j] = mfma<PREC>(fb[gq - a][j], fa[a][i], acc[gq][i][j]);
-            if (CH64 && ks + 1 < KSTEPS && (it * BK + (ks + 1) * KR) % g.chunk_rows == 0) fold();
         }
-        if (CH64 && (((it + 1) * BK) % g.chunk_rows == 0 || it + 1 == ntiles)) fold();
+        // FP64 fold every chunk_rows rows of K (a multiple of the slab) and
+        // after the last slab, counted rather than divided.
+        if constexpr (CH64) {
+            if (++since == per || it + 1 == ntiles) {
+                fold();
+                since = 0;
+            }
+        }
     }
 
     store_tile<CH64, FM, FN>(g, split, m0 + wm * WM, n0 + wn * WN, [&](int i, int j, int r) -> double {
@@ -816,24 +831,42 @@ __global__ void split_kernel(int rows, int cols, const float *X, int64_t ldx, in
     typedef typename Lds<PREC>::T T;
     constexpr int NP = pieces<PREC>();
     const int j = blockIdx.y;
-    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 8;
     const int prow = (rows + 63) / 64 * 64;
     if (i0 >= prow) return;
     const float sc = ldexpf(1.0f, col_exp ? col_exp[j] : a_exp);
     const float *x = X + (int64_t)j * ldx;
     uint16_t *o = out + (int64_t)j * ldo;
+    // Eight rows per thread: two 16-byte loads and one 16-byte store per
+    // piece where the column and its piece rows are aligned.
+    const bool vec = i0 + 8 <= rows && ((((uintptr_t)(x + i0)) & 15) == 0) && ((((uintptr_t)(o + i0)) & 15) == 0) &&
+                     (stride & 7) == 0;
+    float v[8];
+    if (vec) {
+        const floatx4 a = *(const floatx4 *)(x + i0), b = *(const floatx4 *)(x + i0 + 4);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int i = i0 + e;
-        if (i < rows) {
-            T pc[NP];
-            split_value<PREC>(x[i] * sc, pc);
+        for (int e = 0; e < 4; ++e) { v[e] = a[e]; v[4 + e] = b[e]; }
+    } else {
 #pragma unroll
-            for (int c = 0; c < NP; ++c) o[c * stride + i] = __builtin_bit_cast(uint16_t, pc[c]);
-        } else if (i < prow) {
+        for (int e = 0; e < 8; ++e) v[e] = (i0 + e < rows) ? x[i0 + e] : 0.0f;
+    }
+    ushort8 pv[NP];
 #pragma unroll
-            for (int c = 0; c < NP; ++c) o[c * stride + i] = 0;
-        }
+    for (int e = 0; e < 8; ++e) {
+        T pc[NP];
+        split_value<PREC>(v[e] * sc, pc);
+#pragma unroll
+        for (int c = 0; c < NP; ++c) pv[c][e] = (i0 + e < rows) ? __builtin_bit_cast(uint16_t, pc[c]) : (uint16_t)0;
+    }
+    if (vec) {
+#pragma unroll
+        for (int c = 0; c < NP; ++c) *(ushort8 *)(o + c * stride + i0) = pv[c];
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            if (i0 + e < prow)
+#pragma unroll
+                for (int c = 0; c < NP; ++c) o[c * stride + i0 + e] = pv[c][e];
     }
 }
 
@@ -878,7 +911,7 @@ void launch_split(hipStream_t s, int rows, int cols, const float *X, int64_t ldx
         hipLaunchKernelGGL(split_t_kernel<PREC>, grid, dim3(256), 0, s, rows, cols, X, ldx, a_exp, col_exp, out, ldo,
                            stride);
     } else {
-        dim3 grid((unsigned)ceil_div64((rows + 63) / 64 * 64, 1024), (unsigned)cols);
+        dim3 grid((unsigned)ceil_div64((rows + 63) / 64 * 64, 2048), (unsigned)cols);
         hipLaunchKernelGGL(split_kernel<PREC>, grid, dim3(256), 0, s, rows, cols, X, ldx, a_exp, col_exp, out, ldo,
                            stride);
     }
@@ -893,26 +926,33 @@ void launch_one(hipStream_t s, const GemmArgs &g, dim3 grid)
 template <int PREC, bool AKM, bool CH64, bool PRE>
 void launch_tile(hipStream_t s, const GemmArgs &g, int nsplit, bool small)
 {
-    const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
     if constexpr (PRE) {
-        if (!small) {
-            // Eight waves for the three-piece form, whose four-wave tiles
-            // would not fit their accumulators in registers.
+        // Pre-split operands stream through LDS on 128 x 128 tiles; products
+        // the split plan gives 64 x 64 tiles (few tiles, many splits) keep the
+        // register-staged kernel, measured faster there.  The tile size does
+        // not change any element's summation order, so the bits are the same.
+        // Eight waves for the three-piece form, whose four-wave tiles would
+        // not fit their accumulators in registers.
+        if (small) {
+            const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
+            launch_one<PREC, 64, 64, true, CH64, true>(s, g, g64);
+        } else {
             constexpr int NW = PREC == GEMM_F16X6 ? 8 : 4;
             const dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
             hipLaunchKernelGGL((gemm_lds_kernel<PREC, CH64, NW>), grid, dim3(64 * NW), 0, s, g);
-            return;
         }
-    }
-    if constexpr (PREC == GEMM_F16X6) {
-        // Three accumulator groups: a 128x128 tile would spill, so the
-        // three-piece form always runs 64x64 tiles.
-        launch_one<PREC, 64, 64, AKM, CH64, PRE>(s, g, g64);
-    } else if (small) {
-        launch_one<PREC, 64, 64, AKM, CH64, PRE>(s, g, g64);
     } else {
-        dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
-        launch_one<PREC, 128, 128, AKM, CH64, PRE>(s, g, grid);
+        const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
+        if constexpr (PREC == GEMM_F16X6) {
+            // Three accumulator groups: a 128x128 tile would spill, so the
+            // fused three-piece form always runs 64x64 tiles.
+            launch_one<PREC, 64, 64, AKM, CH64, false>(s, g, g64);
+        } else if (small) {
+            launch_one<PREC, 64, 64, AKM, CH64, false>(s, g, g64);
+        } else {
+            dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
+            launch_one<PREC, 128, 128, AKM, CH64, false>(s, g, grid);
+        }
     }
 }
 
