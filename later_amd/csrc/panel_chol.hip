@@ -338,7 +338,36 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
     if (G > kDirect) sum_partials(cw.g, 1, w, Gs);  // reduced by chol_reduce_kernel
     else sum_partials(PASS == 1 ? cw.p1 : cw.p2, G, w, Gs);
     __syncthreads();
-    if (t < 64) {
+    bool first_order = false;
+    if constexpr (PASS == 2) {
+        // Q1 is orthonormal to working accuracy, its Gram is I + E.  When
+        // max |E| <= 1e-5, R2 = I + U and R2^{-1} = I - U with
+        // U = triu(E, 1) + diag(E) / 2 agree with the Cholesky factor and its
+        // inverse to O(w |E|^2) <= 3.2e-9, far below the FP32 rounding of Q;
+        // otherwise (and for a zero column of Q1) the Cholesky runs.  Every
+        // workgroup sees the same E, so all take the same branch.
+        __shared__ float emax_w[4];
+        float e = 0.0f;
+        for (int q = t; q < kW * kW; q += kRows) {
+            const int i = q % kW, j = q / kW;
+            if (i <= j) e = fmaxf(e, (float)fabs(Gs[i][j] - (i == j ? 1.0 : 0.0)));
+        }
+#pragma unroll
+        for (int mask = 32; mask > 0; mask >>= 1) e = fmaxf(e, __shfl_xor(e, mask, 64));
+        if ((t & 63) == 0) emax_w[t >> 6] = e;
+        __syncthreads();
+        const float emax = fmaxf(fmaxf(emax_w[0], emax_w[1]), fmaxf(emax_w[2], emax_w[3]));
+        first_order = emax <= 1e-5f;  // NaN compares false: Cholesky path
+        if (first_order) {
+            for (int q = t; q < kW * kW; q += kRows) {
+                const int i = q / kW, j = q % kW;
+                const double u = (i < j) ? Gs[i][j] : (i == j ? 0.5 * (Gs[i][i] - 1.0) : 0.0);
+                Rs[i][j] = (i == j ? 1.0 : 0.0) + u;
+                Ris[i][j] = (i == j ? 1.0 : 0.0) - u;
+            }
+        }
+    }
+    if (t < 64 && !first_order) {
         const bool bad = cholesky_wave(Gs, Rs, Ris);
         if (bad && t == 0 && blockIdx.x == 0 && status) atomicOr(status, 1);
     }
