@@ -568,8 +568,8 @@ template <int N> __device__ __forceinline__ void wait_vmcnt()
 // lane's 16 bytes land at the wave's LDS base + 16 * lane, no registers).
 // 128 x 128 tiles, four waves of 64 x 64, slabs of 32 k.  In a stage, each
 // piece of each operand tile is 128 rows of 64 bytes whose four 16-byte
-// chunks are XOR-swizzled by (row / 2) % 4; a fragment read (8 rows x one
-// chunk per 8 lanes) then touches 8 distinct bank groups.  NS stages are in
+// chunks are XOR-swizzled by (row / 4) % 4; a fragment read (16 rows x one
+// chunk per 16 lanes) then covers all 64 banks once.  NS stages are in
 // flight: a wave waits on its own loads with s_waitcnt, the barrier publishes
 // everyone's, and the freed stage is refilled right after it.  The piece
 // arrays must be zero-padded in K to a multiple of 32 (rows of A or B past M
@@ -616,7 +616,7 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
                 const int blk = BPW * wave + i;  // 1 KB block of the piece tile
                 const int q = blk * 64 + lane;
                 const int r = q >> 2;
-                const int cg = (q & 3) ^ ((r >> 1) & 3);
+                const int cg = (q & 3) ^ ((r >> 2) & 3);
                 const uint16_t *src = P + (int64_t)min(r0 + r, rmax - 1) * ld + k0 + 8 * cg;
 #pragma unroll
                 for (int c = 0; c < NP; ++c)
@@ -628,7 +628,7 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
         }
     };
     auto frag = [&](const char *pbase, int row, int ks) -> FT {
-        const int ch = (2 * ks + (lane >> 5)) ^ ((row >> 1) & 3);
+        const int ch = (2 * ks + (lane >> 5)) ^ ((row >> 2) & 3);
         return *(const FT *)(pbase + row * 64 + ch * 16);
     };
 
@@ -686,16 +686,23 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
         asm volatile("s_barrier" ::: "memory");
         if (it + NS - 1 < ntiles) issue(it + NS - 1, (it + NS - 1) % NS);
         const char *sb = lds + (it % NS) * SB;
-#pragma unroll
-        for (int ks = 0; ks < KSTEPS; ++ks) {
-            FT fa[NP][FM], fb[NP][FN];
+        // Fragments double-buffered across the slab's k-steps: the reads of
+        // step 1 are in flight while step 0's MFMAs run.
+        FT fa[KSTEPS][NP][FM], fb[KSTEPS][NP][FN];
+        auto load_frags = [&](int ks) {
 #pragma unroll
             for (int c = 0; c < NP; ++c) {
 #pragma unroll
-                for (int i = 0; i < FM; ++i) fa[c][i] = frag(sb + c * PB, wm * WM + 32 * i + (lane & 31), ks);
+                for (int i = 0; i < FM; ++i) fa[ks][c][i] = frag(sb + c * PB, wm * WM + 32 * i + (lane & 31), ks);
 #pragma unroll
-                for (int j = 0; j < FN; ++j) fb[c][j] = frag(sb + (NP + c) * PB, wn * WN + 32 * j + (lane & 31), ks);
+                for (int j = 0; j < FN; ++j)
+                    fb[ks][c][j] = frag(sb + (NP + c) * PB, wn * WN + 32 * j + (lane & 31), ks);
             }
+        };
+        load_frags(0);
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            if (ks + 1 < KSTEPS) load_frags(ks + 1);
 #pragma unroll
             for (int gq = 0; gq < NG; ++gq)
 #pragma unroll
@@ -704,7 +711,7 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
                     for (int i = 0; i < FM; ++i)
 #pragma unroll
                         for (int j = 0; j < FN; ++j)
-                            acc[gq][i][j] = mfma<PREC>(fb[gq - a][j], fa[a][i], acc[gq][i][j]);
+                            acc[gq][i][j] = mfma<PREC>(fb[ks][gq - a][j], fa[ks][a][i], acc[gq][i][j]);
         }
         // FP64 fold every chunk_rows rows of K (a multiple of the slab) and
         // after the last slab, counted rather than divided.

@@ -20,8 +20,11 @@
 // workgroups, up to 256) add their partials in a small reduction kernel
 // before K2 and K3 instead of in every workgroup.
 //
-// One pass of Cholesky QR loses orthogonality like kappa^2 u; the second pass
-// starts from kappa(Q1) ~ 1 and restores O(u) for kappa(panel) up to ~1e7.
+// One pass of Cholesky QR loses orthogonality like kappa^2 u_64 (the Gram and
+// its factor are FP64) plus the FP32 rounding of Q; the second pass starts
+// from kappa(Q1) ~ 1 and restores O(u) for kappa(panel) up to ~1e7.  Panels
+// known to be near-orthonormal (a repeated factorisation of a Q) take one
+// pass (`once`).
 // A pivot that is not positive and finite, or that cancels to below
 // 64 u_64 of the column's squared norm (numerical rank deficiency in the FP64
 // Gram), sets status bit 0 and yields a zero column, like the MGS panel.
@@ -312,6 +315,8 @@ __device__ __forceinline__ void load_block_mfma(const float *A, int64_t lda, int
 //           stored; partials p2 of the stored Q1.
 //   PASS 2: Gram of Q1 -> R2; Q = Q1 R2^{-1} stored; workgroup 0 writes
 //           R = R2 R1 (ldr, FP32).
+//   PASS 0: the only pass (panels with condition number near 1): Gram of A
+//           -> R1; Q = A R1^{-1} stored; workgroup 0 writes R = R1.
 // The product with R^{-1} runs on FP64 MFMA as D = R^{-T} A^T per 16 x 16
 // tile: the A operand (slices of R^{-1}) stays in registers for the whole
 // block, the B operand is the block's rows, and D's lanes walk 16
@@ -336,11 +341,11 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
     load_block_mfma(A, lda, m, w, blockIdx.x * kRows, av);
 
     if (G > kDirect) sum_partials(cw.g, 1, w, Gs);  // reduced by chol_reduce_kernel
-    else sum_partials(PASS == 1 ? cw.p1 : cw.p2, G, w, Gs);
+    else sum_partials(PASS == 2 ? cw.p2 : cw.p1, G, w, Gs);
     __syncthreads();
     bool first_order = false;
-    if constexpr (PASS == 2) {
-        // Q1 is orthonormal to working accuracy, its Gram is I + E.  When
+    if constexpr (PASS != 1) {
+        // The panel (Q1, or a near-orthonormal input) has Gram I + E.  When
         // max |E| <= 1e-5, R2 = I + U and R2^{-1} = I - U with
         // U = triu(E, 1) + diag(E) / 2 agree with the Cholesky factor and its
         // inverse to O(w |E|^2) <= 3.2e-9, far below the FP32 rounding of Q;
@@ -375,6 +380,11 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
     if (blockIdx.x == 0) {
         if constexpr (PASS == 1) {
             for (int e = t; e < kW * kW; e += kRows) cw.r1[e] = Rs[e / kW][e % kW];
+        } else if constexpr (PASS == 0) {
+            for (int e = t; e < w * w; e += kRows) {
+                const int i = e % w, j = e / w;
+                R[i + (int64_t)j * ldr] = (i <= j) ? (float)Rs[i][j] : 0.0f;
+            }
         } else {
             // R = R2 R1 (both upper), FP64 then rounded once; R1 staged in
             // LDS (over the Gram, no longer needed).
@@ -443,7 +453,7 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
 int64_t panel_chol_work_floats() { return ((int64_t)2 * kMaxGroups * kPacked + kW * kW + kPacked) * 2 + 64; }
 
 int panel_cholqr2(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr, float *work, int64_t lwork,
-                  int *status, int *launches)
+                  int *status, int *launches, bool once)
 {
     if (w < 1 || w > kW || m < w) return LATER_ERR_SHAPE;
     if (lwork < panel_chol_work_floats()) return LATER_ERR_WORKSPACE;
@@ -453,6 +463,13 @@ int panel_cholqr2(hipStream_t s, int m, int w, float *A, int lda, float *R, int 
     const dim3 rgrid((kPacked + 31) / 32);
     hipLaunchKernelGGL(chol_gram_kernel, dim3(G), dim3(kRows), 0, s, m, w, (const float *)A, (int64_t)lda, work);
     if (G > kDirect) hipLaunchKernelGGL(chol_reduce_kernel, rgrid, dim3(256), 0, s, cw.p1, G, cw.g);
+    if (once) {
+        hipLaunchKernelGGL(chol_apply_kernel<0>, dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
+                           (int64_t)ldr, status);
+        HIP_OK(hipGetLastError());
+        if (launches) *launches += G > kDirect ? 3 : 2;
+        return LATER_OK;
+    }
     hipLaunchKernelGGL(chol_apply_kernel<1>, dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
                        (int64_t)ldr, status);
     if (G > kDirect) hipLaunchKernelGGL(chol_reduce_kernel, rgrid, dim3(256), 0, s, cw.p2, G, cw.g);

@@ -116,6 +116,7 @@ struct Run {
     int depth;
     bool node_reorth;  // block CGS2 at the nodes of this pass
     int chunk;         // FP64 fold chunk of this pass
+    bool chol_once;    // one Cholesky QR pass per leaf (the input is a Q)
 };
 
 __global__ void zero_lower_kernel(int n, float *R, int64_t ldr)
@@ -161,7 +162,8 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     if (depth > r.depth) r.depth = depth;
     if (n <= 32) {
         if (r.o.panel == LATER_PANEL_CHOLQR2)
-            return panel_cholqr2(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n, r.status, &r.launches);
+            return panel_cholqr2(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n, r.status, &r.launches,
+                                 r.chol_once);
         return panel_tsqr(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n,
                           r.o.panel == LATER_PANEL_MGS64, r.status, &r.launches);
     }
@@ -355,6 +357,7 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     r.depth = 0;
     r.node_reorth = (o.reorth & LATER_REORTH_NODE) != 0;
     r.chunk = o.chunk;
+    r.chol_once = false;
 
     hipLaunchKernelGGL(status_clear_kernel, dim3(1), dim3(64), 0, r.s, r.status);
     hipLaunchKernelGGL(zero_lower_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R, (int64_t)ldr);
@@ -366,6 +369,7 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
         hipLaunchKernelGGL(fill_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, (int64_t)n * n, R2, 0.0f);
         ++r.launches;
         r.node_reorth = false;  // Q has condition number ~1: one projection per node suffices
+        r.chol_once = true;     // and one Cholesky QR pass per leaf
         r.chunk = o.chunk2 > 0 ? o.chunk2 : o.chunk;
         rc = qr_rec(r, m, n, A, lda, R2, n, o.gemm_pass2, 0);
         if (!rc) {
