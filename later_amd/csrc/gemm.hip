@@ -566,38 +566,44 @@ template <int N> __device__ __forceinline__ void wait_vmcnt()
 
 // Pre-split operands streamed straight into LDS (global_load_lds_dwordx4: each
 // lane's 16 bytes land at the wave's LDS base + 16 * lane, no registers).
-// 128 x 128 tiles, four waves of 64 x 64, slabs of 32 k.  In a stage, each
-// piece of each operand tile is 128 rows of 64 bytes whose four 16-byte
-// chunks are XOR-swizzled by (row / 4) % 4; a fragment read (16 rows x one
-// chunk per 16 lanes) then covers all 64 banks once.  NS stages are in
-// flight: a wave waits on its own loads with s_waitcnt, the barrier publishes
-// everyone's, and the freed stage is refilled right after it.  The piece
-// arrays must be zero-padded in K to a multiple of 32 (rows of A or B past M
-// or N are clamped: they only feed outputs that are not stored).
-template <int PREC, bool CH64, int NW>
+// BM x 128 tiles, slabs of 32 k, waves in a WGM x 2 grid of (BM / WGM) x 64
+// wave tiles.  In a stage, each piece of each operand tile is a run of rows
+// of 64 bytes whose four 16-byte chunks are XOR-swizzled by (row / 4) % 4; a
+// fragment read (16 rows x one chunk per 16 lanes) then covers all 64 banks
+// once.  NS stages are in flight: a wave waits on its own loads with
+// s_waitcnt, the barrier publishes everyone's, and the freed stage is refilled
+// right after it.  The piece arrays must be zero-padded in K to a multiple of
+// 32 (rows of A or B past M or N are clamped: they only feed outputs that are
+// not stored).
+template <int PREC, bool CH64, int BM, int WGM>
 __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, const int tn, const int split)
 {
-    // NW = 4: a 2 x 2 grid of 64 x 64 wave tiles; NW = 8: 4 x 2 of 32 x 64
-    // (half the accumulators per wave, for the three-piece form).
-    constexpr int BK = 32, WGM = NW / 2, WM = 128 / WGM, WN = 64, FM = WM / 32, FN = 2, KR = 16, KSTEPS = 2;
-    constexpr int BPW = 8 / NW;               // 1 KB blocks per wave per piece tile
+    // BM 128, WGM 2: four waves of 64 x 64; BM 128, WGM 4: eight waves of
+    // 32 x 64 (half the accumulators per wave, for the three-piece form);
+    // BM 256, WGM 4: eight waves of 64 x 64 (two per SIMD, and a third less
+    // operand traffic per product than the 128 x 128 tile).
+    constexpr int NW = 2 * WGM, BN = 128;
+    constexpr int BK = 32, WM = BM / WGM, WN = 64, FM = WM / 32, FN = 2, KSTEPS = 2;
+    constexpr int BPA = BM / 16 / NW, BPB = BN / 16 / NW;  // 1 KB blocks per wave per piece tile
+    static_assert(BPA >= 1 && BPB >= 1 && FM >= 1, "tile shape");
     constexpr int NP = pieces<PREC>(), NG = NP;
-    constexpr int NS = NP == 3 ? 3 : 4;      // stages: 48 or 32 KB each (X6 / X3), 16 KB for one piece
-    constexpr int PB = 128 * BK * 2;          // bytes of one piece of one operand tile
-    constexpr int SB = 2 * NP * PB;           // bytes of one stage
-    constexpr int LPS = 2 * NP * BPW;         // loads per wave per stage
+    constexpr int PBA = BM * BK * 2, PBB = BN * BK * 2;  // bytes of one piece of one operand tile
+    constexpr int SB = NP * (PBA + PBB);                 // bytes of one stage
+    constexpr int NS = (163840 / SB) < 4 ? (163840 / SB) : 4;
+    static_assert(NS >= 2, "stages");
+    constexpr int LPS = NP * (BPA + BPB);                // loads per wave per stage
     typedef typename Frag<PREC>::T FT;
     __shared__ __attribute__((aligned(16))) char lds[NS * SB];
 
-    const int m0 = tm * 128, n0 = tn * 128;
+    const int m0 = tm * BM, n0 = tn * BN;
     int kbeg = split * g.kper;
     int kend = min(g.K, kbeg + g.kper);
     if (g.tri) {
         // Upper-triangular operands: tiles below the diagonal get nothing and
-        // only k in [m0, n0 + 128) contributes (both multiples of the slab).
-        if (m0 > n0 + 127) return;
+        // only k in [m0, n0 + BN) contributes (both multiples of the slab).
+        if (m0 > n0 + BN - 1) return;
         kbeg = max(kbeg, m0);
-        kend = min(kend, n0 + 128);
+        kend = min(kend, n0 + BN);
     }
     const int ntiles = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -611,9 +617,11 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
             const uint16_t *P = op ? g.Bh : g.Ah;
             const int64_t ld = op ? g.ldbh : g.ldah, ps = op ? g.psb : g.psa;
             const int r0 = op ? n0 : m0, rmax = op ? g.N : g.M;
+            const int bpw = op ? BPB : BPA, pb = op ? PBB : PBA;
+            char *obase = sbase + (op ? NP * PBA : 0);
 #pragma unroll
-            for (int i = 0; i < BPW; ++i) {
-                const int blk = BPW * wave + i;  // 1 KB block of the piece tile
+            for (int i = 0; i < (op ? BPB : BPA); ++i) {
+                const int blk = bpw * wave + i;  // 1 KB block of the piece tile
                 const int q = blk * 64 + lane;
                 const int r = q >> 2;
                 const int cg = (q & 3) ^ ((r >> 2) & 3);
@@ -622,8 +630,7 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
                 for (int c = 0; c < NP; ++c)
                     __builtin_amdgcn_global_load_lds(
                         (const void *)(src + c * ps),
-                        (__attribute__((address_space(3))) void *)(sbase + (op * NP + c) * PB + blk * 1024), 16, 0,
-                        0);
+                        (__attribute__((address_space(3))) void *)(obase + c * pb + blk * 1024), 16, 0, 0);
             }
         }
     };
@@ -693,10 +700,10 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
 #pragma unroll
             for (int c = 0; c < NP; ++c) {
 #pragma unroll
-                for (int i = 0; i < FM; ++i) fa[ks][c][i] = frag(sb + c * PB, wm * WM + 32 * i + (lane & 31), ks);
+                for (int i = 0; i < FM; ++i) fa[ks][c][i] = frag(sb + c * PBA, wm * WM + 32 * i + (lane & 31), ks);
 #pragma unroll
                 for (int j = 0; j < FN; ++j)
-                    fb[ks][c][j] = frag(sb + (NP + c) * PB, wn * WN + 32 * j + (lane & 31), ks);
+                    fb[ks][c][j] = frag(sb + NP * PBA + c * PBB, wn * WN + 32 * j + (lane & 31), ks);
             }
         };
         load_frags(0);
@@ -729,12 +736,12 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
     });
 }
 
-template <int PREC, bool CH64, int NW>
-__global__ __launch_bounds__(64 * NW) void gemm_lds_kernel(GemmArgs g)
+template <int PREC, bool CH64, int BM, int WGM>
+__global__ __launch_bounds__(128 * WGM) void gemm_lds_kernel(GemmArgs g)
 {
     int tm, tn, split;
     tile_of(tm, tn, split);
-    gemm_lds_body<PREC, CH64, NW>(g, tm, tn, split);
+    gemm_lds_body<PREC, CH64, BM, WGM>(g, tm, tn, split);
 }
 
 // Grouped launch: workgroup b computes tile (t.y, t.z) of split t.w of the
@@ -944,9 +951,13 @@ void launch_tile(hipStream_t s, const GemmArgs &g, int nsplit, bool small)
             const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
             launch_one<PREC, 64, 64, true, CH64, true>(s, g, g64);
         } else {
-            constexpr int NW = PREC == GEMM_F16X6 ? 8 : 4;
-            const dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
-            hipLaunchKernelGGL((gemm_lds_kernel<PREC, CH64, NW>), grid, dim3(64 * NW), 0, s, g);
+            // 256-row tiles for the two-piece form without FP64 folds (the
+            // folds' FP64 accumulators leave no registers for a second wave
+            // per SIMD).
+            constexpr int BM = (PREC == GEMM_F16X3 && !CH64) ? 256 : 128;
+            constexpr int WGM = (PREC == GEMM_F16X6 || BM == 256) ? 4 : 2;
+            const dim3 grid((unsigned)ceil_div64(g.M, BM), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
+            hipLaunchKernelGGL((gemm_lds_kernel<PREC, CH64, BM, WGM>), grid, dim3(128 * WGM), 0, s, g);
         }
     } else {
         const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
