@@ -23,6 +23,13 @@ namespace {
 
 constexpr int NT = 256;  // 4 waves in a 2 x 2 grid
 constexpr int kMaxBK = 64;
+// FP64 folds of the split forms (chunk_rows > 0): every chunk_rows rows of K
+// and after the last k, the leading accumulator group (the hi x hi products)
+// is added into the FP64 sums and cleared; every kLeadFolds-th time, and
+// after the last k, the other groups follow.  The lower groups hold products 2^-11 and 2^-22 smaller, so their FP32
+// sums may run kLeadFolds times longer for the same error relative to the
+// result.  Both GEMM kernels fold at the same k, so their bits agree.
+constexpr int kLeadFolds = 8;
 
 // K depth of one LDS slab.  64 for the split-FP16 forms without the FP64
 // fold (their three or six products per slab then cover the next slab's
@@ -429,7 +436,7 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
         if constexpr (NG > 2) v += (double)acc[2][i][j][r] * 0x1p-22;
         return v;
     };
-    auto fold = [&]() {
+    auto fold_lead = [&]() {
         if constexpr (CH64) {
 #pragma unroll
             for (int i = 0; i < FM; ++i)
@@ -437,9 +444,24 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
                 for (int j = 0; j < FN; ++j)
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        acc64[i][j][r] += combined(i, j, r);
+                        acc64[i][j][r] += (double)acc[0][i][j][r];
+                        acc[0][i][j][r] = 0.0f;
+                    }
+        }
+    };
+    auto fold_rest = [&]() {
+        if constexpr (CH64 && NG > 1) {
 #pragma unroll
-                        for (int q = 0; q < NG; ++q) acc[q][i][j][r] = 0.0f;
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        double v = (double)acc[1][i][j][r] * 0x1p-11;
+                        if constexpr (NG > 2) v += (double)acc[2][i][j][r] * 0x1p-22;
+                        acc64[i][j][r] += v;
+#pragma unroll
+                        for (int q = 1; q < NG; ++q) acc[q][i][j][r] = 0.0f;
                     }
         }
     };
@@ -447,7 +469,7 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
     const int ntiles = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
 
     const int per_ks = CH64 ? g.chunk_rows / KR : 1;  // k-steps per FP64 fold
-    int ksince = 0;
+    int ksince = 0, nlead = 0;
     // One slab of MFMAs from LDS buffer `cur` (slab `it` of this split).
     auto compute = [&](int cur, int it) {
 #pragma unroll
@@ -472,12 +494,16 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
 #pragma unroll
                         for (int j = 0; j < FN; ++j)
                             acc[gq][i][j] = mfma<PREC>(fb[gq - a][j], fa[a][i], acc[gq][i][j]);
-            // FP64 fold every chunk_rows rows of K (a multiple of 32) and
-            // after the last k-step, counted rather than divided.
+            // FP64 folds (kLeadFolds), counted rather than divided.
             if constexpr (CH64) {
-                if (++ksince == per_ks || (ks + 1 == KSTEPS && it + 1 == ntiles)) {
-                    fold();
+                const bool last = ks + 1 == KSTEPS && it + 1 == ntiles;
+                if (++ksince == per_ks || last) {
                     ksince = 0;
+                    fold_lead();
+                    if (++nlead == kLeadFolds || last) {
+                        nlead = 0;
+                        fold_rest();
+                    }
                 }
             }
         }
@@ -663,7 +689,7 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
         if constexpr (NG > 2) v += (double)acc[2][i][j][r] * 0x1p-22;
         return v;
     };
-    auto fold = [&]() {
+    auto fold_lead = [&]() {
         if constexpr (CH64) {
 #pragma unroll
             for (int i = 0; i < FM; ++i)
@@ -671,15 +697,30 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
                 for (int j = 0; j < FN; ++j)
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        acc64[i][j][r] += combined(i, j, r);
+                        acc64[i][j][r] += (double)acc[0][i][j][r];
+                        acc[0][i][j][r] = 0.0f;
+                    }
+        }
+    };
+    auto fold_rest = [&]() {
+        if constexpr (CH64 && NG > 1) {
 #pragma unroll
-                        for (int q = 0; q < NG; ++q) acc[q][i][j][r] = 0.0f;
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        double v = (double)acc[1][i][j][r] * 0x1p-11;
+                        if constexpr (NG > 2) v += (double)acc[2][i][j][r] * 0x1p-22;
+                        acc64[i][j][r] += v;
+#pragma unroll
+                        for (int q = 1; q < NG; ++q) acc[q][i][j][r] = 0.0f;
                     }
         }
     };
 
     const int per = CH64 ? g.chunk_rows / BK : 1;  // slabs per FP64 fold
-    int since = 0;
+    int since = 0, nlead = 0;
 #pragma unroll
     for (int st = 0; st < NS - 1; ++st)
         if (st < ntiles) issue(st, st);
@@ -720,12 +761,16 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
                         for (int j = 0; j < FN; ++j)
                             acc[gq][i][j] = mfma<PREC>(fb[ks][gq - a][j], fa[ks][a][i], acc[gq][i][j]);
         }
-        // FP64 fold every chunk_rows rows of K (a multiple of the slab) and
-        // after the last slab, counted rather than divided.
+        // FP64 folds (kLeadFolds), counted rather than divided.
         if constexpr (CH64) {
-            if (++since == per || it + 1 == ntiles) {
-                fold();
+            const bool last = it + 1 == ntiles;
+            if (++since == per || last) {
                 since = 0;
+                fold_lead();
+                if (++nlead == kLeadFolds || last) {
+                    nlead = 0;
+                    fold_rest();
+                }
             }
         }
     }
@@ -941,21 +986,20 @@ template <int PREC, bool AKM, bool CH64, bool PRE>
 void launch_tile(hipStream_t s, const GemmArgs &g, int nsplit, bool small)
 {
     if constexpr (PRE) {
-        // Pre-split operands stream through LDS on 128 x 128 tiles; products
-        // the split plan gives 64 x 64 tiles (few tiles, many splits) keep the
-        // register-staged kernel, measured faster there.  The tile size does
-        // not change any element's summation order, so the bits are the same.
-        // Eight waves for the three-piece form, whose four-wave tiles would
-        // not fit their accumulators in registers.
+        // Pre-split operands stream through LDS on 128- or 256-row tiles;
+        // products the split plan gives 64 x 64 tiles (few tiles, many
+        // splits) keep the register-staged kernel, measured faster there.
+        // The tile size does not change any element's summation order, so
+        // the bits are the same.
         if (small) {
             const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
             launch_one<PREC, 64, 64, true, CH64, true>(s, g, g64);
         } else {
-            // 256-row tiles for the two-piece form without FP64 folds (the
-            // folds' FP64 accumulators leave no registers for a second wave
-            // per SIMD).
+            // The split forms run eight waves (two per SIMD): 256-row tiles
+            // for the two-piece form without FP64 folds, 128-row tiles where
+            // the folds' FP64 accumulators take the registers.
             constexpr int BM = (PREC == GEMM_F16X3 && !CH64) ? 256 : 128;
-            constexpr int WGM = (PREC == GEMM_F16X6 || BM == 256) ? 4 : 2;
+            constexpr int WGM = (PREC == GEMM_F16X6 || PREC == GEMM_F16X3) ? 4 : 2;
             const dim3 grid((unsigned)ceil_div64(g.M, BM), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
             hipLaunchKernelGGL((gemm_lds_kernel<PREC, CH64, BM, WGM>), grid, dim3(128 * WGM), 0, s, g);
         }
