@@ -35,9 +35,10 @@ int64_t panel_work_floats_any(int m);
 int panel_tsqr(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr, float *work,
                int64_t lwork, bool fp64, int *status, int *launches);
 
-// Cholesky QR twice for an m x w panel (w <= 32) in three launches (once:
+// Cholesky QR twice for an m x w panel (w <= 64) in three launches (once:
 // one pass, two launches, for panels with condition number near 1); *status
 // gets bit 0 set on a numerically rank-deficient or non-finite Gram pivot.
-int64_t panel_chol_work_floats();
+// The workspace depends on whether w is above 32.
+int64_t panel_chol_work_floats(int w);
 int panel_cholqr2(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr, float *work, int64_t lwork,
                   int *status, int *launches, bool once = false);

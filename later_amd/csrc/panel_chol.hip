@@ -2,15 +2,16 @@
 //
 // An alternative to the block-MGS TSQR tree for the leaves of the accurate
 // forms, with no sequential loop over columns across the grid.  For an m x w
-// panel (w <= 32), three launches over the same G workgroups (G depends on m
-// only, so every result is bit-reproducible):
+// panel (w <= 64; kernels for widths 32 and 64), three launches over the same
+// G workgroups (G depends on m only, so every result is bit-reproducible):
 //   K1  each workgroup forms the Gram matrix of its rows with FP64 MFMA
 //       (products of FP32 values are exact in FP64) and stores the packed
 //       upper triangle as its partial;
 //   K2  every workgroup adds the G partials in the same fixed order, factors
-//       G = R1^T R1 and inverts R1 (one elimination, FP64, one wave, columns
-//       in registers), forms q = a R1^{-1} for each of its rows in FP64 and
-//       stores Q1 in FP32, then forms the partial Gram of the stored Q1;
+//       G = R1^T R1 and inverts R1 (FP64, one wave per 32 x 32 diagonal
+//       block, columns in registers), forms q = a R1^{-1} for each of its
+//       rows in FP64 and stores Q1 in FP32, then forms the partial Gram of
+//       the stored Q1;
 //   K3  the same for Q1: R2, Q = Q1 R2^{-1}, R = R2 R1.
 // All of the small-matrix arithmetic is FP64, so multiplying by the explicit
 // inverse adds errors of order u_64 kappa, far below the FP32 rounding of Q.
@@ -35,28 +36,49 @@ namespace {
 
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-constexpr int kW = 32;                    // max panel width
+constexpr int kWMax = 64;                 // max panel width
 constexpr int kRows = 256;                // rows per block = threads per workgroup
 constexpr int kMaxGroups = 256;           // workgroups (and Gram partials) per launch
 constexpr int kDirect = 32;               // up to this many partials, every workgroup adds them itself
-constexpr int kPacked = kW * (kW + 1) / 2;  // packed upper triangle of a 32 x 32 Gram
 constexpr double kPivotRel = 64.0 * 2.220446049250313e-16;
 
-struct CholWork {
-    double *p1;  // kMaxGroups x kPacked: partial Grams of A
-    double *p2;  // kMaxGroups x kPacked: partial Grams of Q1
-    double *r1;  // kW x kW, row-major: R1 (upper)
-    double *g;   // kPacked: a reduced Gram (more than kDirect partials)
+// Sizes of the width-W kernels (W = 32 or 64).
+template <int W> struct Shape {
+    static constexpr int CT = W / 16;             // 16-column tiles
+    static constexpr int NT = CT * (CT + 1) / 2;  // distinct 16 x 16 tiles of the symmetric Gram
+    static constexpr int P = W * (W + 1) / 2;     // packed upper triangle
+    static constexpr int LDG = W + 1;             // row stride of the Gram in LDS
+    static constexpr int KS = W / 4;              // MFMA k-steps over the columns
 };
 
-__host__ __device__ inline CholWork carve(float *ws)
+// Index of Gram tile (a, b), a <= b < CT, in row-major upper-triangular order.
+template <int CT> __host__ __device__ constexpr int tile_index(int a, int b)
 {
+    return a * CT - a * (a - 1) / 2 + (b - a);
+}
+
+// Workspace of the width-W kernels (doubles; the partials have stride P).
+struct CholWork {
+    double *p1;  // kMaxGroups x P: partial Grams of A
+    double *p2;  // kMaxGroups x P: partial Grams of Q1
+    double *r1;  // W x W, row-major: R1 (upper)
+    double *g;   // P: a reduced Gram (more than kDirect partials)
+};
+
+template <int W> __host__ __device__ inline CholWork carve(float *ws)
+{
+    constexpr int P = Shape<W>::P;
     CholWork c;
     c.p1 = (double *)ws;
-    c.p2 = c.p1 + (int64_t)kMaxGroups * kPacked;
-    c.r1 = c.p2 + (int64_t)kMaxGroups * kPacked;
-    c.g = c.r1 + kW * kW;
+    c.p2 = c.p1 + (int64_t)kMaxGroups * P;
+    c.r1 = c.p2 + (int64_t)kMaxGroups * P;
+    c.g = c.r1 + W * W;
     return c;
+}
+
+template <int W> constexpr int64_t work_floats()
+{
+    return ((int64_t)2 * kMaxGroups * Shape<W>::P + W * W + Shape<W>::P) * 2 + 64;
 }
 
 // Packed index e of the upper triangle (column-major, i <= j) -> (i, j).
@@ -69,73 +91,85 @@ __device__ __forceinline__ void unpack(int e, int &i, int &j)
     i = e - c * (c + 1) / 2;
 }
 
-// The three distinct 16 x 16 tiles of a 32 x 32 Gram: (0,0), (0,1), (1,1).
-struct Gram3 {
-    doublex4 t00, t01, t11;
+// The distinct 16 x 16 tiles (a, b), a <= b, of a W x W Gram.
+template <int W> struct Gram {
+    doublex4 t[Shape<W>::NT];
 };
+
+template <int W> __device__ __forceinline__ void gram_zero(Gram<W> &g)
+{
+#pragma unroll
+    for (int q = 0; q < Shape<W>::NT; ++q) g.t[q] = doublex4{0.0, 0.0, 0.0, 0.0};
+}
 
 // v_mfma_f64_16x16x4f64: A (16 x 4) lane l holds A[l%16][l/16], B (4 x 16)
 // lane l holds B[l/16][l%16], D (16 x 16) lane l item r holds
-// D[l/16 + 4r][l%16] (rows interleaved, unlike the FP32 16 x 16 forms).  For the Gram, the k index is a row of the panel: the
-// lane's value of column c = l%16 at row k serves as both A[c][k] and B[k][c].
-// a0 holds columns 0..15, a1 columns 16..31, at four rows (one per MFMA).
-__device__ __forceinline__ void gram_step(Gram3 &g, const double a0[4], const double a1[4])
+// D[l/16 + 4r][l%16] (rows interleaved, unlike the FP32 16 x 16 forms).  For
+// the Gram, the k index is a row of the panel: the lane's value of column
+// c = 16 t + l%16 at row k serves as both A[c][k] and B[k][c].  a[t][p] holds
+// column tile t at four rows (one per MFMA).
+template <int W> __device__ __forceinline__ void gram_step(Gram<W> &g, const double (*a)[4])
 {
+    constexpr int CT = Shape<W>::CT;
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        g.t00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[p], a0[p], g.t00, 0, 0, 0);
-        g.t01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[p], a1[p], g.t01, 0, 0, 0);
-        g.t11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[p], a1[p], g.t11, 0, 0, 0);
-    }
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int x = 0; x < CT; ++x)
+#pragma unroll
+            for (int y = x; y < CT; ++y)
+                g.t[tile_index<CT>(x, y)] =
+                    __builtin_amdgcn_mfma_f64_16x16x4f64(a[x][p], a[y][p], g.t[tile_index<CT>(x, y)], 0, 0, 0);
 }
 
 // Adds the four waves' Grams in wave order and stores this workgroup's
-// packed partial.  Cb: 4 x 3 x 4 x 64 doubles of LDS.
-__device__ __forceinline__ void gram_finish(const Gram3 &g, double *Cb, double *part)
+// packed partial.  Cb: 4 x NT x 4 x 64 doubles of LDS.
+template <int W> __device__ __forceinline__ void gram_finish(const Gram<W> &g, double *Cb, double *part)
 {
+    constexpr int NT = Shape<W>::NT, CT = Shape<W>::CT;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const doublex4 *tl[3] = {&g.t00, &g.t01, &g.t11};
 #pragma unroll
-    for (int t = 0; t < 3; ++t)
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Cb[((wv * 3 + t) * 4 + r) * 64 + lane] = (*tl[t])[r];
+        for (int r = 0; r < 4; ++r) Cb[((wv * NT + t) * 4 + r) * 64 + lane] = g.t[t][r];
     __syncthreads();
-    for (int e = threadIdx.x; e < kPacked; e += kRows) {
+    for (int e = threadIdx.x; e < Shape<W>::P; e += kRows) {
         int i, j;
         unpack(e, i, j);
-        const int t = (i >> 4) + (j >> 4);  // (0,0) -> 0, (0,1) -> 1, (1,1) -> 2
+        const int t = tile_index<CT>(i >> 4, j >> 4);
         const int ii = i & 15, jj = j & 15;
         const int ln = jj + 16 * (ii & 3), r = ii >> 2;
-        double s = Cb[((0 * 3 + t) * 4 + r) * 64 + ln];
-        s += Cb[((1 * 3 + t) * 4 + r) * 64 + ln];
-        s += Cb[((2 * 3 + t) * 4 + r) * 64 + ln];
-        s += Cb[((3 * 3 + t) * 4 + r) * 64 + ln];
+        double s = Cb[((0 * NT + t) * 4 + r) * 64 + ln];
+        s += Cb[((1 * NT + t) * 4 + r) * 64 + ln];
+        s += Cb[((2 * NT + t) * 4 + r) * 64 + ln];
+        s += Cb[((3 * NT + t) * 4 + r) * 64 + ln];
         part[e] = s;
     }
 }
 
-// G (LDS, 32 x 33) = sum over g < ngroups of the packed partials, in order
+// G (LDS, W x (W + 1)) = sum over g < ngroups of the packed partials, in order
 // g = 0, 1, ...; entries outside the w x w block are the identity's.
-__device__ __forceinline__ void sum_partials(const double *part, int ngroups, int w, double (*Gs)[kW + 1])
+template <int W>
+__device__ __forceinline__ void sum_partials(const double *part, int ngroups, int w, double *Gs)
 {
-    constexpr int kPer = (kPacked + kRows - 1) / kRows;  // 3
+    constexpr int P = Shape<W>::P, LDG = Shape<W>::LDG;
+    constexpr int kPer = (P + kRows - 1) / kRows;  // 3 or 9
     int ei[kPer];
     double s[kPer];
 #pragma unroll
     for (int x = 0; x < kPer; ++x) {
-        ei[x] = min((int)threadIdx.x + x * kRows, kPacked - 1);
+        ei[x] = min((int)threadIdx.x + x * kRows, P - 1);
         s[x] = 0.0;
     }
-    // Up to 48 loads in flight per thread: the partials were written by other
+    // Many loads in flight per thread: the partials were written by other
     // XCDs and come from memory, so each round costs a full memory latency.
-    constexpr int kBatch = 16;
+    constexpr int kBatch = W == 32 ? 16 : 4;
     for (int g0 = 0; g0 < ngroups; g0 += kBatch) {
         double v[kPer][kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             const int g = min(g0 + u, ngroups - 1);
 #pragma unroll
-            for (int x = 0; x < kPer; ++x) v[x][u] = part[(int64_t)g * kPacked + ei[x]];
+            for (int x = 0; x < kPer; ++x) v[x][u] = part[(int64_t)g * P + ei[x]];
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u)
@@ -146,13 +180,13 @@ __device__ __forceinline__ void sum_partials(const double *part, int ngroups, in
 #pragma unroll
     for (int x = 0; x < kPer; ++x) {
         const int e = threadIdx.x + x * kRows;
-        if (e < kPacked) {
+        if (e < P) {
             int i, j;
             unpack(e, i, j);
             double val = s[x];
             if (i >= w || j >= w) val = (i == j) ? 1.0 : 0.0;
-            Gs[i][j] = val;
-            Gs[j][i] = val;
+            Gs[i * LDG + j] = val;
+            Gs[j * LDG + i] = val;
         }
     }
 }
@@ -165,28 +199,33 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Cholesky G = R^T R of the 32 x 32 Gram in LDS by wave 0, and R^{-1}, in
+// Cholesky S = R^T R of a 32 x 32 block in LDS by one wave, and R^{-1}, in
 // one right-looking elimination, fully unrolled.  Lane j < 32 keeps column j
 // of the trailing matrix in registers; lane 32 + c keeps column c of the
 // identity, to which the same row operations are applied (Gauss-Jordan on
-// [G | I]): it ends as column c of the unit-lower M with M G = D L^T, and
-// R^{-1}[c][k] = M[k][c] / sqrt(d_k).  Column k's multipliers G[i][k] are read
+// [S | I]): it ends as column c of the unit-lower M with M S = D L^T, and
+// R^{-1}[c][k] = M[k][c] / sqrt(d_k).  Column k's multipliers S[i][k] are read
 // from lane k with v_readlane and serve both halves.
-// Writes Rs[k][j] = R[k][j] (upper) and Ris[c][k] = R^{-1}[c][k] (upper).  A
-// bad pivot zeroes row and column k of both.  Returns true if a pivot was bad.
-__device__ __forceinline__ bool cholesky_wave(const double (*Gs)[kW + 1], double (*Rs)[kW], double (*Ris)[kW])
+// S: row stride LDS_; d0[k]: the squared norm of column k that the pivot is
+// compared with; R, Ri (row stride LDR) get R[k][j] (upper) and R^{-1}[c][k]
+// (upper).  A bad pivot k zeroes row k of R and column k of R^{-1} (so column k
+// of Q), and drops row k from the later columns' elimination.  Returns true
+// if a pivot was bad.
+template <int LDS_, int LDR>
+__device__ __forceinline__ bool cholesky_wave(const double *S, const double *d0, double *R, double *Ri)
 {
-    const int lane = threadIdx.x;
+    constexpr int kW = 32;
+    const int lane = threadIdx.x & 63;
     const bool aug = lane >= kW;
     const int j = lane & (kW - 1);
     double g[kW];
 #pragma unroll
-    for (int i = 0; i < kW; ++i) g[i] = aug ? (i == j ? 1.0 : 0.0) : Gs[i][j];
+    for (int i = 0; i < kW; ++i) g[i] = aug ? (i == j ? 1.0 : 0.0) : S[i * LDS_ + j];
     bool bad = false;
 #pragma unroll
     for (int k = 0; k < kW; ++k) {
         const double p = readlane_f64(g[k], k);
-        const double p0 = Gs[k][k];
+        const double p0 = d0[k];
         const bool ok = p0 > 0.0 && p > kPivotRel * p0 && p <= 1.7976931348623157e308;
         bad |= !ok;
         // 1/sqrt(p): hardware estimate and two Newton steps.
@@ -198,9 +237,9 @@ __device__ __forceinline__ bool cholesky_wave(const double (*Gs)[kW + 1], double
         const double d = ok ? p * y : 0.0;
         const double val = g[k] * inv;
         if (aug)
-            Ris[j][k] = val;
+            Ri[j * LDR + k] = val;
         else
-            Rs[k][j] = (j < k) ? 0.0 : (j == k ? d : val);
+            R[k * LDR + j] = (j < k) ? 0.0 : (j == k ? d : val);
         const double a = val * inv;  // row k of the trailing matrix (or of M) / p
 #pragma unroll
         for (int i = k + 1; i < kW; ++i) g[i] = fma(-readlane_f64(g[i], k), a, g[i]);
@@ -208,38 +247,95 @@ __device__ __forceinline__ bool cholesky_wave(const double (*Gs)[kW + 1], double
     return bad;
 }
 
-// Gram MFMA operands of one 16-row group at local row r0 + 4 (l/16) of a
-// column-major LDS block Qs[col][kRows + 4].
-__device__ __forceinline__ void gram_lds(Gram3 &g, const float (*Qs)[kRows + 4], int r0)
+// G = R^T R and R^{-1} for the W x W Gram in LDS (Gs, row stride W + 1; R and
+// Ri row stride W).  W = 64 runs the blocked form with 32 x 32 blocks
+// G = [A B; B^T C]: A = R11^T R11, R12 = R11^{-T} B, C - R12^T R12 = R22^T R22,
+// R^{-1} = [R11^{-1}, -R11^{-1} R12 R22^{-1}; 0, R22^{-1}] (all FP64); a pivot
+// of the second block is compared with the diagonal of C.  Destroys Gs.
+// Called by every thread of the workgroup; returns (on thread 0) whether a
+// pivot was bad.
+template <int W> __device__ __forceinline__ bool factor(double *Gs, double *Rs, double *Ris, double *d0)
 {
-    const int lane = threadIdx.x & 63;
-    const int c = lane & 15, r = r0 + 4 * (lane >> 4);
-    const floatx4 x0 = *(const floatx4 *)&Qs[c][r];
-    const floatx4 x1 = *(const floatx4 *)&Qs[16 + c][r];
-    double a0[4], a1[4];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        a0[p] = (double)x0[p];
-        a1[p] = (double)x1[p];
+    constexpr int LDG = Shape<W>::LDG;
+    const int t = threadIdx.x;
+    if (t < W) d0[t] = Gs[t * LDG + t];
+    __syncthreads();
+    bool bad = false;
+    if (t < 64) bad = cholesky_wave<LDG, W>(Gs, d0, Rs, Ris);
+    if constexpr (W == 64) {
+        __syncthreads();
+        // R12 = R11^{-T} B -> Rs[i][32 + j]; zero lower-left blocks.
+        for (int q = t; q < 32 * 32; q += kRows) {
+            const int i = q >> 5, j = q & 31;
+            double s = 0.0;
+            for (int k = 0; k <= i; ++k) s = fma(Ris[k * W + i], Gs[k * LDG + 32 + j], s);
+            Rs[i * W + 32 + j] = s;
+            Rs[(32 + i) * W + j] = 0.0;
+            Ris[(32 + i) * W + j] = 0.0;
+        }
+        __syncthreads();
+        // S = C - R12^T R12, in place over C.
+        for (int q = t; q < 32 * 32; q += kRows) {
+            const int i = q >> 5, j = q & 31;
+            double s = Gs[(32 + i) * LDG + 32 + j];
+            for (int k = 0; k < 32; ++k) s = fma(-Rs[k * W + 32 + i], Rs[k * W + 32 + j], s);
+            Gs[(32 + i) * LDG + 32 + j] = s;
+        }
+        __syncthreads();
+        if (t < 64) bad |= cholesky_wave<LDG, W>(Gs + 32 * LDG + 32, d0 + 32, Rs + 32 * W + 32, Ris + 32 * W + 32);
+        __syncthreads();
+        // T = R12 R22^{-1} -> over A (Gs[i][j]).
+        for (int q = t; q < 32 * 32; q += kRows) {
+            const int i = q >> 5, j = q & 31;
+            double s = 0.0;
+            for (int k = 0; k <= j; ++k) s = fma(Rs[i * W + 32 + k], Ris[(32 + k) * W + 32 + j], s);
+            Gs[i * LDG + j] = s;
+        }
+        __syncthreads();
+        // R^{-1}[i][32 + j] = -(R11^{-1} T)[i][j].  A bad pivot k leaves, as
+        // in one block, row k of R and column k of R^{-1} zero.
+        for (int q = t; q < 32 * 32; q += kRows) {
+            const int i = q >> 5, j = q & 31;
+            double s = 0.0;
+            for (int k = i; k < 32; ++k) s = fma(Ris[i * W + k], Gs[k * LDG + j], s);
+            Ris[i * W + 32 + j] = -s;
+        }
     }
-    gram_step(g, a0, a1);
+    return bad;
 }
 
-// Sum of more than kDirect partial Grams, in a fixed tree: workgroup w owns
-// entries 32w .. 32w+31; thread (q, e) adds groups q*G/8 .. (q+1)*G/8 - 1 in
+// Gram MFMA operands of one 16-row group at local row r0 + 4 (l/16) of a
+// column-major LDS block Qs[col][kRows + 4].
+template <int W> __device__ __forceinline__ void gram_lds(Gram<W> &g, const float *Qs, int r0)
+{
+    constexpr int CT = Shape<W>::CT;
+    const int lane = threadIdx.x & 63;
+    const int c = lane & 15, r = r0 + 4 * (lane >> 4);
+    double a[CT][4];
+#pragma unroll
+    for (int x = 0; x < CT; ++x) {
+        const floatx4 v = *(const floatx4 *)&Qs[(16 * x + c) * (kRows + 4) + r];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) a[x][p] = (double)v[p];
+    }
+    gram_step<W>(g, a);
+}
+
+// Sum of more than kDirect partial Grams, in a fixed tree: workgroup b owns
+// entries 32b .. 32b+31; thread (q, e) adds groups q*G/8 .. (q+1)*G/8 - 1 in
 // order, then the eight part sums are added in order of q.
-__global__ __launch_bounds__(256) void chol_reduce_kernel(const double *part, int ngroups, double *out)
+template <int W> __global__ __launch_bounds__(256) void chol_reduce_kernel(const double *part, int ngroups, double *out)
 {
     __shared__ double ps[8][32];
     const int e = blockIdx.x * 32 + (threadIdx.x & 31), q = threadIdx.x >> 5;
     const int per = (ngroups + 7) / 8, g0 = q * per, g1 = min(ngroups, g0 + per);
     double s = 0.0;
-    if (e < kPacked) {
+    if (e < Shape<W>::P) {
         constexpr int kB = 16;
         for (int g = g0; g < g1; g += kB) {
             double v[kB];
 #pragma unroll
-            for (int u = 0; u < kB; ++u) v[u] = part[(int64_t)min(g + u, g1 - 1) * kPacked + e];
+            for (int u = 0; u < kB; ++u) v[u] = part[(int64_t)min(g + u, g1 - 1) * Shape<W>::P + e];
 #pragma unroll
             for (int u = 0; u < kB; ++u)
                 if (g + u < g1) s += v[u];
@@ -247,7 +343,7 @@ __global__ __launch_bounds__(256) void chol_reduce_kernel(const double *part, in
     }
     ps[q][threadIdx.x & 31] = s;
     __syncthreads();
-    if (threadIdx.x < 32 && e < kPacked) {
+    if (threadIdx.x < 32 && e < Shape<W>::P) {
         double t = ps[0][threadIdx.x];
 #pragma unroll
         for (int k = 1; k < 8; ++k) t += ps[k][threadIdx.x];
@@ -255,45 +351,50 @@ __global__ __launch_bounds__(256) void chol_reduce_kernel(const double *part, in
     }
 }
 
+template <int W>
 __global__ __launch_bounds__(kRows) void chol_gram_kernel(int m, int w, const float *A, int64_t lda, float *ws)
 {
-    __shared__ double Cb[4 * 3 * 4 * 64];
-    const CholWork cw = carve(ws);
+    constexpr int CT = Shape<W>::CT;
+    __shared__ double Cb[4 * Shape<W>::NT * 4 * 64];
+    const CholWork cw = carve<W>(ws);
     const int nblk = (m + kRows - 1) / kRows;
     const int G = gridDim.x;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int c = lane & 15;
-    const int64_t col0 = (int64_t)min(c, w - 1) * lda, col1 = (int64_t)min(16 + c, w - 1) * lda;
-    const bool ok0 = c < w, ok1 = 16 + c < w;
-    Gram3 g;
-    g.t00 = g.t01 = g.t11 = doublex4{0.0, 0.0, 0.0, 0.0};
+    int64_t col[CT];
+    bool ok[CT];
+#pragma unroll
+    for (int x = 0; x < CT; ++x) {
+        col[x] = (int64_t)min(16 * x + c, w - 1) * lda;
+        ok[x] = 16 * x + c < w;
+    }
+    Gram<W> g;
+    gram_zero<W>(g);
     for (int b = blockIdx.x; b < nblk; b += G) {
 #pragma unroll
         for (int sg = 0; sg < 4; ++sg) {
             const int row0 = b * kRows + wv * 64 + sg * 16 + 4 * (lane >> 4);
-            double a0[4], a1[4];
+            double a[CT][4];
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
                 const int row = row0 + p, rc = min(row, m - 1);
-                const float x0 = A[rc + col0], x1 = A[rc + col1];
-                a0[p] = (row < m && ok0) ? (double)x0 : 0.0;
-                a1[p] = (row < m && ok1) ? (double)x1 : 0.0;
+#pragma unroll
+                for (int x = 0; x < CT; ++x) {
+                    const float v = A[rc + col[x]];
+                    a[x][p] = (row < m && ok[x]) ? (double)v : 0.0;
+                }
             }
-            gram_step(g, a0, a1);
+            gram_step<W>(g, a);
         }
     }
-    gram_finish(g, Cb, cw.p1 + (int64_t)blockIdx.x * kPacked);
+    gram_finish<W>(g, Cb, cw.p1 + (int64_t)blockIdx.x * Shape<W>::P);
 }
 
-// One Cholesky QR pass over the rows of this workgroup.
-//   PASS 1: partials p1 (of A) -> R1 (stored to r1 by workgroup 0); Q1 = A R1^{-1}
-//           stored; partials p2 of the stored Q1.
-//   PASS 2: partials p2 -> R2; Q = Q1 R2^{-1} stored; workgroup 0 writes
-//           R = R2 R1 (ldr, FP32).
 // Rows of one 256-row block in the operand layout of the apply MFMA: wave wv
 // owns rows 64 wv .. 64 wv + 63 as four 16-row tiles; lane l holds, for tile
 // t and k-step s, A[r0 + 64 wv + 16 t + l%16][4 s + l/16] (zero past m or w).
-__device__ __forceinline__ void load_block_mfma(const float *A, int64_t lda, int m, int w, int r0, float av[4][8])
+template <int W>
+__device__ __forceinline__ void load_block_mfma(const float *A, int64_t lda, int m, int w, int r0, float (*av)[W / 4])
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int kk = lane >> 4;
@@ -302,13 +403,22 @@ __device__ __forceinline__ void load_block_mfma(const float *A, int64_t lda, int
         const int row = r0 + wv * 64 + t * 16 + (lane & 15);
         const int rc = min(row, m - 1);
 #pragma unroll
-        for (int s2 = 0; s2 < 8; ++s2) {
+        for (int s2 = 0; s2 < W / 4; ++s2) {
             const int col = 4 * s2 + kk;
             const float x = A[rc + (int64_t)min(col, w - 1) * lda];
             av[t][s2] = (row < m && col < w) ? x : 0.0f;
         }
     }
 }
+
+// LDS of the apply kernel: the factor's matrices while it is formed, then
+// (PASS 1) the block of stored Q1 rows, then the Gram reduction buffer.
+template <int W> struct ApplyLds {
+    static constexpr int kFactor = (W * Shape<W>::LDG + 2 * W * W + W) * 8;
+    static constexpr int kQs = W * (kRows + 4) * 4;
+    static constexpr int kCb = 4 * Shape<W>::NT * 4 * 64 * 8;
+    static constexpr int kBytes = kFactor > kQs ? (kFactor > kCb ? kFactor : kCb) : (kQs > kCb ? kQs : kCb);
+};
 
 // One Cholesky QR pass over the rows of this workgroup.
 //   PASS 1: Gram of A -> R1 (stored to r1 by workgroup 0); Q1 = A R1^{-1}
@@ -321,41 +431,44 @@ __device__ __forceinline__ void load_block_mfma(const float *A, int64_t lda, int
 // tile: the A operand (slices of R^{-1}) stays in registers for the whole
 // block, the B operand is the block's rows, and D's lanes walk 16
 // consecutive rows of one column (64-byte stores).
-template <int PASS>
+template <int PASS, int W>
 __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *A, int64_t lda, float *ws, float *R,
                                                           int64_t ldr, int *status)
 {
-    __shared__ double Gs[kW][kW + 1];
-    __shared__ double Rs[kW][kW];
-    __shared__ double Ris[kW][kW];
-    __shared__ float Qs[PASS == 1 ? kW : 1][kRows + 4];
-    __shared__ double Cb[PASS == 1 ? 4 * 3 * 4 * 64 : 1];
-    const CholWork cw = carve(ws);
+    constexpr int CT = Shape<W>::CT, KS = Shape<W>::KS, LDG = Shape<W>::LDG;
+    __shared__ __attribute__((aligned(16))) char smem[ApplyLds<W>::kBytes];
+    double *Gs = (double *)smem;      // W x (W + 1)
+    double *Rs = Gs + W * LDG;        // W x W
+    double *Ris = Rs + W * W;         // W x W
+    double *d0 = Ris + W * W;         // W
+    float *Qs = (float *)smem;        // W x (kRows + 4), after the factor
+    double *Cb = (double *)smem;      // after the last block
+    const CholWork cw = carve<W>(ws);
     const int nblk = (m + kRows - 1) / kRows;
     const int G = gridDim.x;
     const int t = threadIdx.x;
     const int lane = t & 63, wv = t >> 6;
 
     // The first block's rows are in flight while the factor is formed.
-    float av[4][8];
-    load_block_mfma(A, lda, m, w, blockIdx.x * kRows, av);
+    float av[4][KS];
+    load_block_mfma<W>(A, lda, m, w, blockIdx.x * kRows, av);
 
-    if (G > kDirect) sum_partials(cw.g, 1, w, Gs);  // reduced by chol_reduce_kernel
-    else sum_partials(PASS == 2 ? cw.p2 : cw.p1, G, w, Gs);
+    if (G > kDirect) sum_partials<W>(cw.g, 1, w, Gs);  // reduced by chol_reduce_kernel
+    else sum_partials<W>(PASS == 2 ? cw.p2 : cw.p1, G, w, Gs);
     __syncthreads();
     bool first_order = false;
     if constexpr (PASS != 1) {
         // The panel (Q1, or a near-orthonormal input) has Gram I + E.  When
         // max |E| <= 1e-5, R2 = I + U and R2^{-1} = I - U with
         // U = triu(E, 1) + diag(E) / 2 agree with the Cholesky factor and its
-        // inverse to O(w |E|^2) <= 3.2e-9, far below the FP32 rounding of Q;
+        // inverse to O(w |E|^2) <= 6.4e-9, far below the FP32 rounding of Q;
         // otherwise (and for a zero column of Q1) the Cholesky runs.  Every
         // workgroup sees the same E, so all take the same branch.
         __shared__ float emax_w[4];
         float e = 0.0f;
-        for (int q = t; q < kW * kW; q += kRows) {
-            const int i = q % kW, j = q / kW;
-            if (i <= j) e = fmaxf(e, (float)fabs(Gs[i][j] - (i == j ? 1.0 : 0.0)));
+        for (int q = t; q < W * W; q += kRows) {
+            const int i = q % W, j = q / W;
+            if (i <= j) e = fmaxf(e, (float)fabs(Gs[i * LDG + j] - (i == j ? 1.0 : 0.0)));
         }
 #pragma unroll
         for (int mask = 32; mask > 0; mask >>= 1) e = fmaxf(e, __shfl_xor(e, mask, 64));
@@ -364,64 +477,64 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
         const float emax = fmaxf(fmaxf(emax_w[0], emax_w[1]), fmaxf(emax_w[2], emax_w[3]));
         first_order = emax <= 1e-5f;  // NaN compares false: Cholesky path
         if (first_order) {
-            for (int q = t; q < kW * kW; q += kRows) {
-                const int i = q / kW, j = q % kW;
-                const double u = (i < j) ? Gs[i][j] : (i == j ? 0.5 * (Gs[i][i] - 1.0) : 0.0);
-                Rs[i][j] = (i == j ? 1.0 : 0.0) + u;
-                Ris[i][j] = (i == j ? 1.0 : 0.0) - u;
+            for (int q = t; q < W * W; q += kRows) {
+                const int i = q / W, j = q % W;
+                const double u = (i < j) ? Gs[i * LDG + j] : (i == j ? 0.5 * (Gs[i * LDG + i] - 1.0) : 0.0);
+                Rs[i * W + j] = (i == j ? 1.0 : 0.0) + u;
+                Ris[i * W + j] = (i == j ? 1.0 : 0.0) - u;
             }
         }
     }
-    if (t < 64 && !first_order) {
-        const bool bad = cholesky_wave(Gs, Rs, Ris);
+    if (!first_order) {
+        const bool bad = factor<W>(Gs, Rs, Ris, d0);
         if (bad && t == 0 && blockIdx.x == 0 && status) atomicOr(status, 1);
     }
     __syncthreads();
     if (blockIdx.x == 0) {
         if constexpr (PASS == 1) {
-            for (int e = t; e < kW * kW; e += kRows) cw.r1[e] = Rs[e / kW][e % kW];
+            for (int e = t; e < W * W; e += kRows) cw.r1[e] = Rs[e];
         } else if constexpr (PASS == 0) {
             for (int e = t; e < w * w; e += kRows) {
                 const int i = e % w, j = e / w;
-                R[i + (int64_t)j * ldr] = (i <= j) ? (float)Rs[i][j] : 0.0f;
+                R[i + (int64_t)j * ldr] = (i <= j) ? (float)Rs[i * W + j] : 0.0f;
             }
         } else {
             // R = R2 R1 (both upper), FP64 then rounded once; R1 staged in
             // LDS (over the Gram, no longer needed).
-            for (int e = t; e < kW * kW; e += kRows) Gs[e / kW][e % kW] = cw.r1[e];
+            for (int e = t; e < W * W; e += kRows) Gs[(e / W) * LDG + e % W] = cw.r1[e];
             __syncthreads();
             for (int e = t; e < w * w; e += kRows) {
                 const int i = e % w, j = e / w;
                 double s = 0.0;
-                for (int k = i; k <= j; ++k) s = fma(Rs[i][k], Gs[k][j], s);
+                for (int k = i; k <= j; ++k) s = fma(Rs[i * W + k], Gs[k * LDG + j], s);
                 R[i + (int64_t)j * ldr] = (i <= j) ? (float)s : 0.0f;
             }
         }
     }
 
     // A-operand slices of R^{-1}: rv[ct][s] = R^{-1}[4 s + l/16][16 ct + l%16]
-    // (zero for k > column, so column tile 0 needs k-steps 0..3 only).
-    double rv[2][8];
+    // (zero for k > column, so column tile ct needs k-steps 0 .. 4 ct + 3).
+    double rv[CT][KS];
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
+    for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-        for (int s2 = 0; s2 < 8; ++s2) rv[ct][s2] = Ris[4 * s2 + (lane >> 4)][16 * ct + (lane & 15)];
+        for (int s2 = 0; s2 < 4 * (ct + 1); ++s2) rv[ct][s2] = Ris[(4 * s2 + (lane >> 4)) * W + 16 * ct + (lane & 15)];
 
-    Gram3 g;
-    g.t00 = g.t01 = g.t11 = doublex4{0.0, 0.0, 0.0, 0.0};
+    Gram<W> g;
+    gram_zero<W>(g);
 #pragma unroll 1
     for (int b = blockIdx.x; b < nblk; b += G) {
-        __syncthreads();  // Qs of the previous block has been read
+        __syncthreads();  // the factor, or Qs of the previous block, has been read
         const int r0 = b * kRows;
 #pragma unroll
         for (int tt = 0; tt < 4; ++tt) {
             const int lrow = wv * 64 + tt * 16 + (lane & 15);  // row within the block
             const int row = r0 + lrow;
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct) {
+            for (int ct = 0; ct < CT; ++ct) {
                 doublex4 d = doublex4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int s2 = 0; s2 < (ct == 0 ? 4 : 8); ++s2)
+                for (int s2 = 0; s2 < 4 * (ct + 1); ++s2)
                     d = __builtin_amdgcn_mfma_f64_16x16x4f64(rv[ct][s2], (double)av[tt][s2], d, 0, 0, 0);
                 // d[r] = Q[row][16 ct + l/16 + 4 r]
 #pragma unroll
@@ -430,52 +543,58 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
                     const bool ok = row < m && col < w;
                     const float q = ok ? (float)d[r] : 0.0f;
                     if (ok) A[row + (int64_t)col * lda] = q;
-                    if constexpr (PASS == 1) Qs[col][lrow] = q;
+                    if constexpr (PASS == 1) Qs[col * (kRows + 4) + lrow] = q;
                 }
             }
         }
-        if (b + G < nblk) load_block_mfma(A, lda, m, w, (b + G) * kRows, av);
+        if (b + G < nblk) load_block_mfma<W>(A, lda, m, w, (b + G) * kRows, av);
         if constexpr (PASS == 1) {
             // Gram of the stored (FP32) Q1 rows of this block.
             __syncthreads();
 #pragma unroll
-            for (int sg = 0; sg < 4; ++sg) gram_lds(g, Qs, wv * 64 + sg * 16);
+            for (int sg = 0; sg < 4; ++sg) gram_lds<W>(g, Qs, wv * 64 + sg * 16);
         }
     }
     if constexpr (PASS == 1) {
         __syncthreads();
-        gram_finish(g, Cb, cw.p2 + (int64_t)blockIdx.x * kPacked);
+        gram_finish<W>(g, Cb, cw.p2 + (int64_t)blockIdx.x * Shape<W>::P);
     }
+}
+
+template <int W>
+void launch_cholqr(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr, float *work, int *status,
+                   int G, bool once)
+{
+    const CholWork cw = carve<W>(work);
+    const dim3 rgrid((Shape<W>::P + 31) / 32);
+    hipLaunchKernelGGL(chol_gram_kernel<W>, dim3(G), dim3(kRows), 0, s, m, w, (const float *)A, (int64_t)lda, work);
+    if (G > kDirect) hipLaunchKernelGGL(chol_reduce_kernel<W>, rgrid, dim3(256), 0, s, cw.p1, G, cw.g);
+    if (once) {
+        hipLaunchKernelGGL((chol_apply_kernel<0, W>), dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
+                           (int64_t)ldr, status);
+        return;
+    }
+    hipLaunchKernelGGL((chol_apply_kernel<1, W>), dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
+                       (int64_t)ldr, status);
+    if (G > kDirect) hipLaunchKernelGGL(chol_reduce_kernel<W>, rgrid, dim3(256), 0, s, cw.p2, G, cw.g);
+    hipLaunchKernelGGL((chol_apply_kernel<2, W>), dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
+                       (int64_t)ldr, status);
 }
 
 } // namespace
 
-int64_t panel_chol_work_floats() { return ((int64_t)2 * kMaxGroups * kPacked + kW * kW + kPacked) * 2 + 64; }
+int64_t panel_chol_work_floats(int w) { return w <= 32 ? work_floats<32>() : work_floats<64>(); }
 
 int panel_cholqr2(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr, float *work, int64_t lwork,
                   int *status, int *launches, bool once)
 {
-    if (w < 1 || w > kW || m < w) return LATER_ERR_SHAPE;
-    if (lwork < panel_chol_work_floats()) return LATER_ERR_WORKSPACE;
+    if (w < 1 || w > kWMax || m < w) return LATER_ERR_SHAPE;
+    if (lwork < panel_chol_work_floats(w)) return LATER_ERR_WORKSPACE;
     const int nblk = (m + kRows - 1) / kRows;
     const int G = nblk < kMaxGroups ? nblk : kMaxGroups;
-    const CholWork cw = carve(work);
-    const dim3 rgrid((kPacked + 31) / 32);
-    hipLaunchKernelGGL(chol_gram_kernel, dim3(G), dim3(kRows), 0, s, m, w, (const float *)A, (int64_t)lda, work);
-    if (G > kDirect) hipLaunchKernelGGL(chol_reduce_kernel, rgrid, dim3(256), 0, s, cw.p1, G, cw.g);
-    if (once) {
-        hipLaunchKernelGGL(chol_apply_kernel<0>, dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
-                           (int64_t)ldr, status);
-        HIP_OK(hipGetLastError());
-        if (launches) *launches += G > kDirect ? 3 : 2;
-        return LATER_OK;
-    }
-    hipLaunchKernelGGL(chol_apply_kernel<1>, dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
-                       (int64_t)ldr, status);
-    if (G > kDirect) hipLaunchKernelGGL(chol_reduce_kernel, rgrid, dim3(256), 0, s, cw.p2, G, cw.g);
-    hipLaunchKernelGGL(chol_apply_kernel<2>, dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
-                       (int64_t)ldr, status);
+    if (w <= 32) launch_cholqr<32>(s, m, w, A, lda, R, ldr, work, status, G, once);
+    else launch_cholqr<64>(s, m, w, A, lda, R, ldr, work, status, G, once);
     HIP_OK(hipGetLastError());
-    if (launches) *launches += G > kDirect ? 5 : 3;
+    if (launches) *launches += (once ? 2 : 3) + (G > kDirect ? (once ? 1 : 2) : 0);
     return LATER_OK;
 }

@@ -78,7 +78,7 @@ Layout layout(int m, int n, const later_opts &o)
     Layout L;
     L.header = kHeaderFloats;
     L.exps = ((int64_t)n + 63) / 64 * 64;
-    L.panel = (o.panel == LATER_PANEL_CHOLQR2) ? panel_chol_work_floats() : panel_work_floats_any(m);
+    L.panel = (o.panel == LATER_PANEL_CHOLQR2) ? panel_chol_work_floats(n) : panel_work_floats_any(m);
     int64_t g = 0;
     walk_gemm(m, n, o.nmin, o.chunk, &g);
     if (o.reorth & LATER_REORTH_PASS) walk_gemm(m, n, o.nmin, o.chunk2 > 0 ? o.chunk2 : o.chunk, &g);
@@ -160,7 +160,9 @@ dim3 grid1(int64_t n) { return dim3((unsigned)ceil_div64(n, 256)); }
 int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, int prec, int depth)
 {
     if (depth > r.depth) r.depth = depth;
-    if (n <= 32) {
+    // Cholesky QR leaves take up to 64 columns (one Gram and factor for what
+    // would otherwise be two leaves and a node); the MGS panels take 32.
+    if (n <= (r.o.panel == LATER_PANEL_CHOLQR2 ? 64 : 32)) {
         if (r.o.panel == LATER_PANEL_CHOLQR2)
             return panel_cholqr2(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n, r.status, &r.launches,
                                  r.chol_once);
@@ -431,7 +433,7 @@ LATER_API int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, 
 {
     if (w < 1 || w > 32 || m < w || lda < m || ldr < w || !A || !R) return LATER_ERR_SHAPE;
     if (fp64_sums < 0 || fp64_sums > 2) return LATER_ERR_OPTION;
-    const int64_t need = kHeaderFloats + (fp64_sums == 2 ? panel_chol_work_floats() : panel_work_floats(m, w));
+    const int64_t need = kHeaderFloats + (fp64_sums == 2 ? panel_chol_work_floats(w) : panel_work_floats(m, w));
     if (!work || lwork < need) return LATER_ERR_WORKSPACE;
     hipStream_t s = ctx_stream(ctxt);
     hipLaunchKernelGGL(status_clear_kernel, dim3(1), dim3(64), 0, s, (int *)work);

@@ -363,6 +363,54 @@ def test_cholqr2_panel_ill_conditioned(later, oracle, m, w):
     assert np.array_equal(Rg, np.triu(Rg))
 
 
+def _graded(m, n, decades, seed=0):
+    """A = U diag(s) V^T with s geometric from 1 to 10^-decades."""
+    rng = np.random.default_rng(seed)
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = 10.0 ** (-decades * np.arange(n) / max(n - 1, 1))
+    return np.asfortranarray((U * s) @ V.T, dtype=np.float32)
+
+
+@pytest.mark.parametrize("m,n", [(4096, 64), (300, 50), (100000, 64), (1000, 33), (64, 64)])
+def test_cholqr2_wide_leaf(later, oracle, m, n):
+    """One Cholesky QR leaf of 33..64 columns (the blocked 64 x 64 factor and
+    inverse) in the mixed form, kappa = 1e6: Householder-level orthogonality,
+    backward error and R against LAPACK in FP64."""
+    A = _graded(m, n, 6)
+    Q, R, info = _run(later, A, "mixed")
+    Q64, R64 = oracle.lapack_qr64(A)
+    assert oracle.orth(Q) < 2e-6 * np.sqrt(n)
+    assert oracle.backward(A, Q, R) < 1e-6
+    assert np.array_equal(R, np.triu(R)) and np.all(np.diag(R) > 0)
+    kappa = np.linalg.cond(A.astype(np.float64))
+    assert oracle.rel_err(R, R64) < 2e-6 * max(1.0, kappa / 100)
+
+
+@pytest.mark.parametrize("dup", [(3, 4), (40, 41), (10, 45)])
+def test_cholqr2_wide_leaf_breakdown(later, oracle, dup):
+    """A repeated column in the first block, in the second block, and across
+    the two: the status bit is set, the dependent column of Q is zero, the
+    others stay orthonormal, and A = QR still holds (R keeps the column's
+    projections on the earlier ones)."""
+    m, n = 2000, 64
+    A = oracle.gen_uniform(m, n, 0, 0)
+    A[:, dup[1]] = A[:, dup[0]]
+    Ad = _dev(later, A)
+    o = later.opts("mixed")
+    work = later.workspace(m, n, o)
+    R, info = later.rgsqrf(Ad, o=o, work=work, info=True)
+    _sync()
+    assert info.status == 0  # the call itself succeeded
+    assert later.read_status(work) & 1
+    Q, Rg = _np(Ad), _np(R)
+    keep = [j for j in range(n) if j != dup[1]]
+    assert np.all(Q[:, dup[1]] == 0)
+    assert np.isfinite(Q).all() and np.isfinite(Rg).all()
+    assert oracle.orth(Q[:, keep]) < 2e-6 * np.sqrt(n)
+    assert oracle.backward(A, Q, Rg) < 1e-6
+
+
 def test_cholqr2_panel_breakdown(later, oracle):
     import torch
     A = oracle.gen_uniform(2000, 16, 0, 0)
