@@ -37,7 +37,9 @@ namespace {
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWMax = 64;                 // max panel width
-constexpr int kRows = 256;                // rows per block = threads per workgroup
+constexpr int kThreads = 256;             // threads per workgroup (four waves)
+constexpr int kTT = 1;                    // 16-row tiles per wave in a block
+constexpr int kRows = 64 * kTT;           // rows per block (workgroup b takes blocks b, b + G, ...)
 constexpr int kMaxGroups = 256;           // workgroups (and Gram partials) per launch
 constexpr int kDirect = 32;               // up to this many partials, every workgroup adds them itself
 constexpr double kPivotRel = 64.0 * 2.220446049250313e-16;
@@ -132,7 +134,7 @@ template <int W> __device__ __forceinline__ void gram_finish(const Gram<W> &g, d
 #pragma unroll
         for (int r = 0; r < 4; ++r) Cb[((wv * NT + t) * 4 + r) * 64 + lane] = g.t[t][r];
     __syncthreads();
-    for (int e = threadIdx.x; e < Shape<W>::P; e += kRows) {
+    for (int e = threadIdx.x; e < Shape<W>::P; e += kThreads) {
         int i, j;
         unpack(e, i, j);
         const int t = tile_index<CT>(i >> 4, j >> 4);
@@ -152,12 +154,12 @@ template <int W>
 __device__ __forceinline__ void sum_partials(const double *part, int ngroups, int w, double *Gs)
 {
     constexpr int P = Shape<W>::P, LDG = Shape<W>::LDG;
-    constexpr int kPer = (P + kRows - 1) / kRows;  // 3 or 9
+    constexpr int kPer = (P + kThreads - 1) / kThreads;  // 3 or 9
     int ei[kPer];
     double s[kPer];
 #pragma unroll
     for (int x = 0; x < kPer; ++x) {
-        ei[x] = min((int)threadIdx.x + x * kRows, P - 1);
+        ei[x] = min((int)threadIdx.x + x * kThreads, P - 1);
         s[x] = 0.0;
     }
     // Many loads in flight per thread: the partials were written by other
@@ -179,7 +181,7 @@ __device__ __forceinline__ void sum_partials(const double *part, int ngroups, in
     }
 #pragma unroll
     for (int x = 0; x < kPer; ++x) {
-        const int e = threadIdx.x + x * kRows;
+        const int e = threadIdx.x + x * kThreads;
         if (e < P) {
             int i, j;
             unpack(e, i, j);
@@ -265,7 +267,7 @@ template <int W> __device__ __forceinline__ bool factor(double *Gs, double *Rs, 
     if constexpr (W == 64) {
         __syncthreads();
         // R12 = R11^{-T} B -> Rs[i][32 + j]; zero lower-left blocks.
-        for (int q = t; q < 32 * 32; q += kRows) {
+        for (int q = t; q < 32 * 32; q += kThreads) {
             const int i = q >> 5, j = q & 31;
             double s = 0.0;
             for (int k = 0; k <= i; ++k) s = fma(Ris[k * W + i], Gs[k * LDG + 32 + j], s);
@@ -275,7 +277,7 @@ template <int W> __device__ __forceinline__ bool factor(double *Gs, double *Rs, 
         }
         __syncthreads();
         // S = C - R12^T R12, in place over C.
-        for (int q = t; q < 32 * 32; q += kRows) {
+        for (int q = t; q < 32 * 32; q += kThreads) {
             const int i = q >> 5, j = q & 31;
             double s = Gs[(32 + i) * LDG + 32 + j];
             for (int k = 0; k < 32; ++k) s = fma(-Rs[k * W + 32 + i], Rs[k * W + 32 + j], s);
@@ -285,7 +287,7 @@ template <int W> __device__ __forceinline__ bool factor(double *Gs, double *Rs, 
         if (t < 64) bad |= cholesky_wave<LDG, W>(Gs + 32 * LDG + 32, d0 + 32, Rs + 32 * W + 32, Ris + 32 * W + 32);
         __syncthreads();
         // T = R12 R22^{-1} -> over A (Gs[i][j]).
-        for (int q = t; q < 32 * 32; q += kRows) {
+        for (int q = t; q < 32 * 32; q += kThreads) {
             const int i = q >> 5, j = q & 31;
             double s = 0.0;
             for (int k = 0; k <= j; ++k) s = fma(Rs[i * W + 32 + k], Ris[(32 + k) * W + 32 + j], s);
@@ -294,7 +296,7 @@ template <int W> __device__ __forceinline__ bool factor(double *Gs, double *Rs, 
         __syncthreads();
         // R^{-1}[i][32 + j] = -(R11^{-1} T)[i][j].  A bad pivot k leaves, as
         // in one block, row k of R and column k of R^{-1} zero.
-        for (int q = t; q < 32 * 32; q += kRows) {
+        for (int q = t; q < 32 * 32; q += kThreads) {
             const int i = q >> 5, j = q & 31;
             double s = 0.0;
             for (int k = i; k < 32; ++k) s = fma(Ris[i * W + k], Gs[k * LDG + j], s);
@@ -352,7 +354,7 @@ template <int W> __global__ __launch_bounds__(256) void chol_reduce_kernel(const
 }
 
 template <int W>
-__global__ __launch_bounds__(kRows) void chol_gram_kernel(int m, int w, const float *A, int64_t lda, float *ws)
+__global__ __launch_bounds__(kThreads) void chol_gram_kernel(int m, int w, const float *A, int64_t lda, float *ws)
 {
     constexpr int CT = Shape<W>::CT;
     __shared__ double Cb[4 * Shape<W>::NT * 4 * 64];
@@ -372,8 +374,8 @@ __global__ __launch_bounds__(kRows) void chol_gram_kernel(int m, int w, const fl
     gram_zero<W>(g);
     for (int b = blockIdx.x; b < nblk; b += G) {
 #pragma unroll
-        for (int sg = 0; sg < 4; ++sg) {
-            const int row0 = b * kRows + wv * 64 + sg * 16 + 4 * (lane >> 4);
+        for (int sg = 0; sg < kTT; ++sg) {
+            const int row0 = b * kRows + (wv * kTT + sg) * 16 + 4 * (lane >> 4);
             double a[CT][4];
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
@@ -390,17 +392,17 @@ __global__ __launch_bounds__(kRows) void chol_gram_kernel(int m, int w, const fl
     gram_finish<W>(g, Cb, cw.p1 + (int64_t)blockIdx.x * Shape<W>::P);
 }
 
-// Rows of one 256-row block in the operand layout of the apply MFMA: wave wv
-// owns rows 64 wv .. 64 wv + 63 as four 16-row tiles; lane l holds, for tile
-// t and k-step s, A[r0 + 64 wv + 16 t + l%16][4 s + l/16] (zero past m or w).
+// Rows of one block in the operand layout of the apply MFMA: wave wv owns
+// kTT 16-row tiles; lane l holds, for tile t and k-step s,
+// A[r0 + 16 (kTT wv + t) + l%16][4 s + l/16] (zero past m or w).
 template <int W>
 __device__ __forceinline__ void load_block_mfma(const float *A, int64_t lda, int m, int w, int r0, float (*av)[W / 4])
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int kk = lane >> 4;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int row = r0 + wv * 64 + t * 16 + (lane & 15);
+    for (int t = 0; t < kTT; ++t) {
+        const int row = r0 + (wv * kTT + t) * 16 + (lane & 15);
         const int rc = min(row, m - 1);
 #pragma unroll
         for (int s2 = 0; s2 < W / 4; ++s2) {
@@ -423,16 +425,15 @@ template <int W> struct ApplyLds {
 // One Cholesky QR pass over the rows of this workgroup.
 //   PASS 1: Gram of A -> R1 (stored to r1 by workgroup 0); Q1 = A R1^{-1}
 //           stored; partials p2 of the stored Q1.
-//   PASS 2: Gram of Q1 -> R2; Q = Q1 R2^{-1} stored; workgroup 0 writes
-//           R = R2 R1 (ldr, FP32).
+//   PASS 2: Gram of Q1 -> R2; Q = Q1 R2^{-1} stored; R = R2 R1 (ldr, FP32).
 //   PASS 0: the only pass (panels with condition number near 1): Gram of A
-//           -> R1; Q = A R1^{-1} stored; workgroup 0 writes R = R1.
+//           -> R1; Q = A R1^{-1} stored; R = R1.
 // The product with R^{-1} runs on FP64 MFMA as D = R^{-T} A^T per 16 x 16
 // tile: the A operand (slices of R^{-1}) stays in registers for the whole
 // block, the B operand is the block's rows, and D's lanes walk 16
 // consecutive rows of one column (64-byte stores).
 template <int PASS, int W>
-__global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *A, int64_t lda, float *ws, float *R,
+__global__ __launch_bounds__(kThreads) void chol_apply_kernel(int m, int w, float *A, int64_t lda, float *ws, float *R,
                                                           int64_t ldr, int *status)
 {
     constexpr int CT = Shape<W>::CT, KS = Shape<W>::KS, LDG = Shape<W>::LDG;
@@ -450,7 +451,7 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
     const int lane = t & 63, wv = t >> 6;
 
     // The first block's rows are in flight while the factor is formed.
-    float av[4][KS];
+    float av[kTT][KS];
     load_block_mfma<W>(A, lda, m, w, blockIdx.x * kRows, av);
 
     if (G > kDirect) sum_partials<W>(cw.g, 1, w, Gs);  // reduced by chol_reduce_kernel
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
         // workgroup sees the same E, so all take the same branch.
         __shared__ float emax_w[4];
         float e = 0.0f;
-        for (int q = t; q < W * W; q += kRows) {
+        for (int q = t; q < W * W; q += kThreads) {
             const int i = q % W, j = q / W;
             if (i <= j) e = fmaxf(e, (float)fabs(Gs[i * LDG + j] - (i == j ? 1.0 : 0.0)));
         }
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
         const float emax = fmaxf(fmaxf(emax_w[0], emax_w[1]), fmaxf(emax_w[2], emax_w[3]));
         first_order = emax <= 1e-5f;  // NaN compares false: Cholesky path
         if (first_order) {
-            for (int q = t; q < W * W; q += kRows) {
+            for (int q = t; q < W * W; q += kThreads) {
                 const int i = q / W, j = q % W;
                 const double u = (i < j) ? Gs[i * LDG + j] : (i == j ? 0.5 * (Gs[i * LDG + i] - 1.0) : 0.0);
                 Rs[i * W + j] = (i == j ? 1.0 : 0.0) + u;
@@ -490,25 +491,30 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
         if (bad && t == 0 && blockIdx.x == 0 && status) atomicOr(status, 1);
     }
     __syncthreads();
-    if (blockIdx.x == 0) {
-        if constexpr (PASS == 1) {
-            for (int e = t; e < W * W; e += kRows) cw.r1[e] = Rs[e];
-        } else if constexpr (PASS == 0) {
-            for (int e = t; e < w * w; e += kRows) {
-                const int i = e % w, j = e / w;
-                R[i + (int64_t)j * ldr] = (i <= j) ? (float)Rs[i * W + j] : 0.0f;
+    if constexpr (PASS == 1) {
+        if (blockIdx.x == 0)
+            for (int e = t; e < W * W; e += kThreads) cw.r1[e] = Rs[e];
+    } else {
+        // R (FP32, ldr): R1 itself (PASS 0) or R2 R1 (PASS 2, both upper,
+        // FP64 then rounded once), four columns per round, column j by
+        // workgroup (j / 4) mod G, with column j of R1 staged in LDS (over
+        // the Gram, no longer needed).
+        for (int j0 = blockIdx.x * 4; j0 < w; j0 += G * 4) {
+            const int c = t >> 6, j = j0 + c, i = t & 63;
+            if constexpr (PASS == 2) {
+                if (j < w && i < W) Gs[c * LDG + i] = (i <= j) ? cw.r1[i * W + j] : 0.0;
+                __syncthreads();
             }
-        } else {
-            // R = R2 R1 (both upper), FP64 then rounded once; R1 staged in
-            // LDS (over the Gram, no longer needed).
-            for (int e = t; e < W * W; e += kRows) Gs[(e / W) * LDG + e % W] = cw.r1[e];
-            __syncthreads();
-            for (int e = t; e < w * w; e += kRows) {
-                const int i = e % w, j = e / w;
+            if (j < w && i < w) {
                 double s = 0.0;
-                for (int k = i; k <= j; ++k) s = fma(Rs[i * W + k], Gs[k * LDG + j], s);
+                if constexpr (PASS == 2) {
+                    for (int k = i; k <= j; ++k) s = fma(Rs[i * W + k], Gs[c * LDG + k], s);
+                } else {
+                    s = Rs[i * W + j];
+                }
                 R[i + (int64_t)j * ldr] = (i <= j) ? (float)s : 0.0f;
             }
+            if constexpr (PASS == 2) __syncthreads();
         }
     }
 
@@ -527,8 +533,8 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
         __syncthreads();  // the factor, or Qs of the previous block, has been read
         const int r0 = b * kRows;
 #pragma unroll
-        for (int tt = 0; tt < 4; ++tt) {
-            const int lrow = wv * 64 + tt * 16 + (lane & 15);  // row within the block
+        for (int tt = 0; tt < kTT; ++tt) {
+            const int lrow = (wv * kTT + tt) * 16 + (lane & 15);  // row within the block
             const int row = r0 + lrow;
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) {
@@ -552,7 +558,7 @@ __global__ __launch_bounds__(kRows) void chol_apply_kernel(int m, int w, float *
             // Gram of the stored (FP32) Q1 rows of this block.
             __syncthreads();
 #pragma unroll
-            for (int sg = 0; sg < 4; ++sg) gram_lds<W>(g, Qs, wv * 64 + sg * 16);
+            for (int sg = 0; sg < kTT; ++sg) gram_lds<W>(g, Qs, (wv * kTT + sg) * 16);
         }
     }
     if constexpr (PASS == 1) {
@@ -567,17 +573,17 @@ void launch_cholqr(hipStream_t s, int m, int w, float *A, int lda, float *R, int
 {
     const CholWork cw = carve<W>(work);
     const dim3 rgrid((Shape<W>::P + 31) / 32);
-    hipLaunchKernelGGL(chol_gram_kernel<W>, dim3(G), dim3(kRows), 0, s, m, w, (const float *)A, (int64_t)lda, work);
+    hipLaunchKernelGGL(chol_gram_kernel<W>, dim3(G), dim3(kThreads), 0, s, m, w, (const float *)A, (int64_t)lda, work);
     if (G > kDirect) hipLaunchKernelGGL(chol_reduce_kernel<W>, rgrid, dim3(256), 0, s, cw.p1, G, cw.g);
     if (once) {
-        hipLaunchKernelGGL((chol_apply_kernel<0, W>), dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
+        hipLaunchKernelGGL((chol_apply_kernel<0, W>), dim3(G), dim3(kThreads), 0, s, m, w, A, (int64_t)lda, work, R,
                            (int64_t)ldr, status);
         return;
     }
-    hipLaunchKernelGGL((chol_apply_kernel<1, W>), dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
+    hipLaunchKernelGGL((chol_apply_kernel<1, W>), dim3(G), dim3(kThreads), 0, s, m, w, A, (int64_t)lda, work, R,
                        (int64_t)ldr, status);
     if (G > kDirect) hipLaunchKernelGGL(chol_reduce_kernel<W>, rgrid, dim3(256), 0, s, cw.p2, G, cw.g);
-    hipLaunchKernelGGL((chol_apply_kernel<2, W>), dim3(G), dim3(kRows), 0, s, m, w, A, (int64_t)lda, work, R,
+    hipLaunchKernelGGL((chol_apply_kernel<2, W>), dim3(G), dim3(kThreads), 0, s, m, w, A, (int64_t)lda, work, R,
                        (int64_t)ldr, status);
 }
 
