@@ -987,11 +987,10 @@ void launch_tile(hipStream_t s, const GemmArgs &g, int nsplit, bool small)
 {
     if constexpr (PRE) {
         // Pre-split operands stream through LDS on 128- or 256-row tiles;
-        // products the split plan gives 64 x 64 tiles (few tiles, many
-        // splits) keep the register-staged kernel, measured faster there.
-        // The tile size does not change any element's summation order, so
-        // the bits are the same.
-        if (small) {
+        // thin products (a side of 64 or less) keep the register-staged
+        // kernel on 64 x 64 tiles.  The tile size does not change any
+        // element's summation order, so the bits are the same.
+        if (small && (g.M <= 64 || g.N <= 64)) {
             const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
             launch_one<PREC, 64, 64, true, CH64, true>(s, g, g64);
         } else {
@@ -1088,17 +1087,23 @@ int colexp_grouped(hipStream_t s, const ColexpArgs *args, const int2 *tasks, int
 // summation order (and the result bits) never depends on the caller's
 // workspace: enough workgroups to cover the CUs twice, at least 256 rows of K
 // per split, and a partial buffer of at most 1/8 of the operands' size.
+// Split-K plan of a projection: 64 x 64 tiles for thin or few-tile products
+// (small), and enough K splits for ~512 tiles x splits, within m / 256
+// splits and a partials budget of m (n1 + n2) / 2 floats (m (n1 + n2) / 8
+// for thin products).  Tiles are counted at 128 x 128 unless a side is 64 or
+// less, so the plan (and every element's summation order) is the same
+// whichever kernel runs it.
 static void proj_plan(int m, int n1, int n2, int chunk, bool *small, int *nsplit, int *kper)
 {
     const int64_t tiles128 = ceil_div64(n1, 128) * ceil_div64(n2, 128);
-    *small = (n1 <= 64 || n2 <= 64 || tiles128 < 128);
-    const int bm = *small ? 64 : 128;
-    const int64_t tiles = ceil_div64(n1, bm) * ceil_div64(n2, bm);
+    const bool thin = n1 <= 64 || n2 <= 64;
+    *small = thin || tiles128 < 128;
+    const int64_t tiles = thin ? ceil_div64(n1, 64) * ceil_div64(n2, 64) : tiles128;
     int64_t ns = ceil_div64(512, tiles);
     const int64_t maxs = m / 256 > 1 ? m / 256 : 1;
     if (ns > maxs) ns = maxs;
     const int64_t per = (int64_t)n1 * n2 * (chunk > 0 ? 2 : 1);
-    const int64_t budget = (int64_t)m * (n1 + n2) / 8 / per;
+    const int64_t budget = (int64_t)m * (n1 + n2) / (thin ? 8 : 2) / per;
     if (ns > budget) ns = budget;
     if (ns < 1) ns = 1;
     int64_t kp = ceil_div64(m, ns);
