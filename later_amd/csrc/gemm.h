@@ -56,7 +56,9 @@ int gemm_pieces(int prec);
 // Splits X (rows x cols FP32, column-major, ldx), scaled by 2^a_exp (col_exp
 // null) or 2^col_exp[j], into the MFMA input pieces of `prec`: piece c at
 // out + c * stride, column-major with ld `ldo` (trans false: element (i, j)
-// at j * ldo + i) or transposed (element (i, j) at i * ldo + j).
+// at j * ldo + i) or transposed (element (i, j) at i * ldo + j).  out, ldo
+// and stride are multiples of 8 elements; the transposed form also writes
+// the zero padding of j up to a multiple of 64 (ldo covers it).
 int gemm_split(hipStream_t s, int prec, int rows, int cols, const float *X, int64_t ldx, int a_exp,
                const int *col_exp, uint16_t *out, int64_t ldo, int64_t stride, bool trans, int *launches);
 

@@ -949,14 +949,23 @@ __global__ __launch_bounds__(256) void split_t_kernel(int rows, int cols, const 
         tile[jj][tx] = v;
     }
     __syncthreads();
+    // Thread (ii, jq): 16 consecutive j of row i0 + ii, two 16-byte stores per
+    // piece (columns cols .. roundup(cols, 64) - 1 are the zero K padding).
+    const int ii = threadIdx.x >> 2, jq = (threadIdx.x & 3) * 16, i = i0 + ii;
+    if (i < rows) {
+        ushort8 pv[NP][2];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int ii = ty + 4 * q, i = i0 + ii, j = j0 + tx;
-        if (i < rows) {  // columns cols .. roundup(cols, 64) - 1: the zero K padding
+        for (int u = 0; u < 16; ++u) {
             T pc[NP];
-            split_value<PREC>(tile[tx][ii], pc);
+            split_value<PREC>(tile[jq + u][ii], pc);
 #pragma unroll
-            for (int c = 0; c < NP; ++c) out[c * stride + (int64_t)i * ldo + j] = __builtin_bit_cast(uint16_t, pc[c]);
+            for (int c = 0; c < NP; ++c) pv[c][u >> 3][u & 7] = __builtin_bit_cast(uint16_t, pc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < NP; ++c) {
+            ushort8 *dst = (ushort8 *)(out + c * stride + (int64_t)i * ldo + j0 + jq);
+            dst[0] = pv[c][0];
+            dst[1] = pv[c][1];
         }
     }
 }

@@ -119,12 +119,15 @@ struct Run {
     bool chol_once;    // one Cholesky QR pass per leaf (the input is a Q)
 };
 
+// Column-wise helpers on a (ceil(rows / 256), min(cols, 65535)) grid: thread
+// row i of columns j = blockIdx.y, blockIdx.y + gridDim.y, ...
+dim3 grid_cols(int rows, int cols) { return dim3((unsigned)ceil_div64(rows, 256), (unsigned)(cols < 65535 ? cols : 65535)); }
+
 __global__ void zero_lower_kernel(int n, float *R, int64_t ldr)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)n * n) return;
-    const int i = (int)(idx % n), j = (int)(idx / n);
-    if (i > j) R[i + (int64_t)j * ldr] = 0.0f;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int j = blockIdx.y; j < n && j < i; j += gridDim.y) R[i + (int64_t)j * ldr] = 0.0f;
 }
 
 __global__ void fill_kernel(int64_t n, float *x, float v)
@@ -147,12 +150,13 @@ __global__ void add_kernel(int m, int n, const float *S, int64_t lds, float *C, 
 // Y (n x n, ld n) = X (upper triangle, ldx); diag(Y) -= shift.
 __global__ void copy_upper_kernel(int n, const float *X, int64_t ldx, float *Y, float shift)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)n * n) return;
-    const int i = (int)(idx % n), j = (int)(idx / n);
-    float v = (i <= j) ? X[i + (int64_t)j * ldx] : 0.0f;
-    if (i == j) v -= shift;
-    Y[idx] = v;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int j = blockIdx.y; j < n; j += gridDim.y) {
+        float v = (i <= j) ? X[i + (int64_t)j * ldx] : 0.0f;
+        if (i == j) v -= shift;
+        Y[i + (int64_t)j * n] = v;
+    }
 }
 
 dim3 grid1(int64_t n) { return dim3((unsigned)ceil_div64(n, 256)); }
@@ -362,7 +366,7 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     r.chol_once = false;
 
     hipLaunchKernelGGL(status_clear_kernel, dim3(1), dim3(64), 0, r.s, r.status);
-    hipLaunchKernelGGL(zero_lower_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R, (int64_t)ldr);
+    hipLaunchKernelGGL(zero_lower_kernel, grid_cols(n, n), dim3(256), 0, r.s, n, R, (int64_t)ldr);
     r.launches += 2;
     rc = hipGetLastError() == hipSuccess ? LATER_OK : LATER_ERR_DEVICE;
     if (!rc) rc = qr_rec(r, m, n, A, lda, R, ldr, o.gemm, 0);
@@ -375,9 +379,9 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
         r.chunk = o.chunk2 > 0 ? o.chunk2 : o.chunk;
         rc = qr_rec(r, m, n, A, lda, R2, n, o.gemm_pass2, 0);
         if (!rc) {
-            hipLaunchKernelGGL(copy_upper_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R, (int64_t)ldr, T,
+            hipLaunchKernelGGL(copy_upper_kernel, grid_cols(n, n), dim3(256), 0, r.s, n, R, (int64_t)ldr, T,
                                0.0f);
-            hipLaunchKernelGGL(copy_upper_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, n, R2, (int64_t)n, R2,
+            hipLaunchKernelGGL(copy_upper_kernel, grid_cols(n, n), dim3(256), 0, r.s, n, R2, (int64_t)n, R2,
                                1.0f);
             r.launches += 2;
             const int p2 = o.gemm_pass2, pu = update_prec(p2);
