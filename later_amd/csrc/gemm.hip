@@ -781,11 +781,30 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
     });
 }
 
+// Tile order of a triangular product (one split).  Tile row tm has work
+// proportional to the K range [m0, n0 + BN), so the rows are far from equal;
+// tile_of's contiguous range per XCD would give XCD 0 the heaviest eight rows.
+// Here XCD x (workgroups L with L % 8 == x) takes rows 16 b + x and
+// 16 b + 15 - x of every block of 16 rows, heavier blocks first.  Returns
+// false (use tile_of) unless the tile rows are a multiple of 16.
+__device__ __forceinline__ bool tile_of_tri(int &tm, int &tn, int &split)
+{
+    const int nx = gridDim.x, ny = gridDim.y;
+    if ((nx & 15) || gridDim.z != 1) return false;
+    const int L = blockIdx.x + nx * blockIdx.y;
+    const int x = L & 7, r = L >> 3;
+    const int b = r / (2 * ny), w = r % (2 * ny);
+    tn = w % ny;
+    tm = 16 * b + (w < ny ? x : 15 - x);
+    split = 0;
+    return true;
+}
+
 template <int PREC, bool CH64, int BM, int WGM>
 __global__ __launch_bounds__(128 * WGM) void gemm_lds_kernel(GemmArgs g)
 {
     int tm, tn, split;
-    tile_of(tm, tn, split);
+    if (!(g.tri && tile_of_tri(tm, tn, split))) tile_of(tm, tn, split);
     gemm_lds_body<PREC, CH64, BM, WGM>(g, tm, tn, split);
 }
 
