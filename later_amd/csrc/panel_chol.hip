@@ -266,42 +266,68 @@ template <int W> __device__ __forceinline__ bool factor(double *Gs, double *Rs, 
     if (t < 64) bad = cholesky_wave<LDG, W>(Gs, d0, Rs, Ris);
     if constexpr (W == 64) {
         __syncthreads();
+        // The four 32 x 32 products below: thread t makes column j = t % 32
+        // of rows i = t / 32 + 8 u, u < 4, over the full k range (the
+        // triangular factors' zero halves are stored, so the sums are the
+        // same), with the shared operand loaded once per k and every load
+        // independent of the running sums.
+        const int j = t & 31, i0 = t >> 5;
+        double s[4];
         // R12 = R11^{-T} B -> Rs[i][32 + j]; zero lower-left blocks.
-        for (int q = t; q < 32 * 32; q += kThreads) {
-            const int i = q >> 5, j = q & 31;
-            double s = 0.0;
-            for (int k = 0; k <= i; ++k) s = fma(Ris[k * W + i], Gs[k * LDG + 32 + j], s);
-            Rs[i * W + 32 + j] = s;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s[u] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const double b = Gs[k * LDG + 32 + j];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s[u] = fma(Ris[k * W + i0 + 8 * u], b, s[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + 8 * u;
+            Rs[i * W + 32 + j] = s[u];
             Rs[(32 + i) * W + j] = 0.0;
             Ris[(32 + i) * W + j] = 0.0;
         }
         __syncthreads();
         // S = C - R12^T R12, in place over C.
-        for (int q = t; q < 32 * 32; q += kThreads) {
-            const int i = q >> 5, j = q & 31;
-            double s = Gs[(32 + i) * LDG + 32 + j];
-            for (int k = 0; k < 32; ++k) s = fma(-Rs[k * W + 32 + i], Rs[k * W + 32 + j], s);
-            Gs[(32 + i) * LDG + 32 + j] = s;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s[u] = Gs[(32 + i0 + 8 * u) * LDG + 32 + j];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const double b = Rs[k * W + 32 + j];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s[u] = fma(-Rs[k * W + 32 + i0 + 8 * u], b, s[u]);
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) Gs[(32 + i0 + 8 * u) * LDG + 32 + j] = s[u];
         __syncthreads();
         if (t < 64) bad |= cholesky_wave<LDG, W>(Gs + 32 * LDG + 32, d0 + 32, Rs + 32 * W + 32, Ris + 32 * W + 32);
         __syncthreads();
         // T = R12 R22^{-1} -> over A (Gs[i][j]).
-        for (int q = t; q < 32 * 32; q += kThreads) {
-            const int i = q >> 5, j = q & 31;
-            double s = 0.0;
-            for (int k = 0; k <= j; ++k) s = fma(Rs[i * W + 32 + k], Ris[(32 + k) * W + 32 + j], s);
-            Gs[i * LDG + j] = s;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s[u] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const double b = Ris[(32 + k) * W + 32 + j];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s[u] = fma(Rs[(i0 + 8 * u) * W + 32 + k], b, s[u]);
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) Gs[(i0 + 8 * u) * LDG + j] = s[u];
         __syncthreads();
         // R^{-1}[i][32 + j] = -(R11^{-1} T)[i][j].  A bad pivot k leaves, as
         // in one block, row k of R and column k of R^{-1} zero.
-        for (int q = t; q < 32 * 32; q += kThreads) {
-            const int i = q >> 5, j = q & 31;
-            double s = 0.0;
-            for (int k = i; k < 32; ++k) s = fma(Ris[i * W + k], Gs[k * LDG + j], s);
-            Ris[i * W + 32 + j] = -s;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s[u] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const double b = Gs[k * LDG + j];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s[u] = fma(Ris[(i0 + 8 * u) * W + k], b, s[u]);
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) Ris[(i0 + 8 * u) * W + 32 + j] = -s[u];
     }
     return bad;
 }
