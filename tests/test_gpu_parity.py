@@ -213,6 +213,27 @@ def test_rgsqrf_graded_mixed_meets_gates(later, oracle):
     assert oracle.orth(Qf) > 1e-4  # one pass of block CGS loses orthogonality here
 
 
+def test_rgsqrf_mixed_square_4096(later, oracle):
+    """A square mixed-form factorisation whose R2 R1 composition has 16 tile
+    rows of 256, so it runs the XCD-balanced triangular tile order: the gates
+    (FP64 metrics on the device, themselves checked against the oracle) and
+    the same bits on a second run."""
+    import torch
+    n = 4096
+    A0 = later.generate_normal(n, n, 0, 0)
+    o = later.opts("mixed")
+    out = []
+    for _ in range(2):
+        A = A0.clone()
+        R = later.rgsqrf(A, o=o)
+        _sync()
+        out.append((A, R))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+    Q, R = out[0]
+    assert later.check_orth(Q) < 1e-5
+    assert later.check_backward(A0, Q, R) < 1e-6
+
+
 def test_rgsqrf_deterministic(later, oracle):
     A = oracle.gen_uniform(4096, 512, 0, 0)
     for form in ("reference", "fp32", "mixed"):
