@@ -534,7 +534,10 @@ __global__ __launch_bounds__(kThreads) void chol_apply_kernel(int m, int w, floa
             if (j < w && i < w) {
                 double s = 0.0;
                 if constexpr (PASS == 2) {
-                    for (int k = i; k <= j; ++k) s = fma(Rs[i * W + k], Gs[c * LDG + k], s);
+                    // Full k range: R2's lower and the staged column's
+                    // below-j entries are zeros, so the sum is the same.
+#pragma unroll
+                    for (int k = 0; k < W; ++k) s = fma(Rs[i * W + k], Gs[c * LDG + k], s);
                 } else {
                     s = Rs[i * W + j];
                 }
