@@ -62,6 +62,12 @@ int gemm_pieces(int prec);
 int gemm_split(hipStream_t s, int prec, int rows, int cols, const float *X, int64_t ldx, int a_exp,
                const int *col_exp, uint16_t *out, int64_t ldo, int64_t stride, bool trans, int *launches);
 
+// gemm_colexp and gemm_split (not transposed) of the same X in one launch:
+// col_exp[j] for each column and the pieces scaled by 2^col_exp[j]
+// (scaled precisions only).
+int gemm_split_exp(hipStream_t s, int prec, int rows, int cols, const float *X, int64_t ldx, int *col_exp,
+                   uint16_t *out, int64_t ldo, int64_t stride, int *launches);
+
 // C (n1 x n2) = Q^T B.  a_exp scales Q by 2^a_exp before the cast; b_exp
 // (per column of B, may be null) likewise.  chunk > 0 folds FP32 MFMA
 // accumulators into FP64 every `chunk` rows of K and reduces split-K in FP64.

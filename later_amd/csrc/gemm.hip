@@ -846,11 +846,11 @@ __global__ void splitk_reduce_kernel(int M, int N, int nsplit, const void *parti
 // Per-column exponents e_j with max_i |X_ij| 2^e_j in [2^14, 2^15) (0 for a
 // zero column).  One 256-thread workgroup per column, 16-byte loads where the
 // column is aligned (the maximum does not depend on the order).
-__global__ __launch_bounds__(256) void colexp_kernel(int m, int n, const float *X, int64_t ldx, int *e)
+// Exponent of column x (m values) by the whole 256-thread workgroup:
+// e with max |x| 2^e in [2^14, 2^15) (0 for a zero or non-finite maximum).
+__device__ __forceinline__ int column_exp(int m, const float *x)
 {
     __shared__ float wmax[4];
-    const int j = blockIdx.x;
-    const float *x = X + (int64_t)j * ldx;
     float mx = 0.0f;
     const int head = (int)min((int64_t)m, (int64_t)((16 - ((uintptr_t)x & 15)) & 15) / 4);
     for (int i = threadIdx.x; i < head; i += 256) mx = fmaxf(mx, fabsf(x[i]));
@@ -866,15 +866,20 @@ __global__ __launch_bounds__(256) void colexp_kernel(int m, int n, const float *
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
     __syncthreads();
     mx = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-    if (threadIdx.x == 0) {
-        int ex = 0;
-        if (mx > 0.0f && mx < INFINITY) {
-            int fe;
-            frexpf(mx, &fe);  // mx in [2^(fe-1), 2^fe)
-            ex = 15 - fe;
-        }
-        e[j] = ex;
+    int ex = 0;
+    if (mx > 0.0f && mx < INFINITY) {
+        int fe;
+        frexpf(mx, &fe);  // mx in [2^(fe-1), 2^fe)
+        ex = 15 - fe;
     }
+    return ex;
+}
+
+__global__ __launch_bounds__(256) void colexp_kernel(int m, int n, const float *X, int64_t ldx, int *e)
+{
+    const int j = blockIdx.x;
+    const int ex = column_exp(m, X + (int64_t)j * ldx);
+    if (threadIdx.x == 0) e[j] = ex;
 }
 
 // Grouped colexp: workgroup b handles column t.y of matrix args[t.x].
@@ -902,21 +907,15 @@ __global__ void colexp_grouped_kernel(const ColexpArgs *args, const int2 *tasks)
 // thread converts 4 consecutive rows of one column.
 // Rows rows .. roundup(rows, 64) - 1 (the K padding the streaming GEMM
 // reads) are written as zeros.
+// Eight rows i0 .. i0 + 7 of column x into its pieces o + c * stride: two
+// 16-byte loads and one 16-byte store per piece where the column and its
+// piece rows are aligned; rows rows .. prow - 1 are written as zeros.
 template <int PREC>
-__global__ void split_kernel(int rows, int cols, const float *X, int64_t ldx, int a_exp, const int *col_exp,
-                             uint16_t *out, int64_t ldo, int64_t stride)
+__device__ __forceinline__ void split_rows8(const float *x, uint16_t *o, int rows, int prow, int64_t stride, int i0,
+                                            float sc)
 {
     typedef typename Lds<PREC>::T T;
     constexpr int NP = pieces<PREC>();
-    const int j = blockIdx.y;
-    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 8;
-    const int prow = (rows + 63) / 64 * 64;
-    if (i0 >= prow) return;
-    const float sc = ldexpf(1.0f, col_exp ? col_exp[j] : a_exp);
-    const float *x = X + (int64_t)j * ldx;
-    uint16_t *o = out + (int64_t)j * ldo;
-    // Eight rows per thread: two 16-byte loads and one 16-byte store per
-    // piece where the column and its piece rows are aligned.
     const bool vec = i0 + 8 <= rows && ((((uintptr_t)(x + i0)) & 15) == 0) && ((((uintptr_t)(o + i0)) & 15) == 0) &&
                      (stride & 7) == 0;
     float v[8];
@@ -946,6 +945,39 @@ __global__ void split_kernel(int rows, int cols, const float *X, int64_t ldx, in
 #pragma unroll
                 for (int c = 0; c < NP; ++c) o[c * stride + i0 + e] = pv[c][e];
     }
+}
+
+// Pieces of X scaled by 2^e (e = a_exp or col_exp[j]), same layout: each
+// thread converts 8 consecutive rows of one column.
+// Rows rows .. roundup(rows, 64) - 1 (the K padding the streaming GEMM
+// reads) are written as zeros.
+template <int PREC>
+__global__ void split_kernel(int rows, int cols, const float *X, int64_t ldx, int a_exp, const int *col_exp,
+                             uint16_t *out, int64_t ldo, int64_t stride)
+{
+    const int j = blockIdx.y;
+    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 8;
+    const int prow = (rows + 63) / 64 * 64;
+    if (i0 >= prow) return;
+    const float sc = ldexpf(1.0f, col_exp ? col_exp[j] : a_exp);
+    split_rows8<PREC>(X + (int64_t)j * ldx, out + (int64_t)j * ldo, rows, prow, stride, i0, sc);
+}
+
+// The column exponents (as colexp_kernel) and the pieces scaled by them (as
+// split_kernel) in one pass per column: workgroup j finds column j's maximum,
+// writes e[j], then splits the column, whose second read comes from cache.
+template <int PREC>
+__global__ __launch_bounds__(256) void split_exp_kernel(int rows, const float *X, int64_t ldx, int *col_exp,
+                                                        uint16_t *out, int64_t ldo, int64_t stride)
+{
+    const int j = blockIdx.x;
+    const float *x = X + (int64_t)j * ldx;
+    const int ex = column_exp(rows, x);
+    if (threadIdx.x == 0) col_exp[j] = ex;
+    const float sc = ldexpf(1.0f, ex);
+    const int prow = (rows + 63) / 64 * 64;
+    for (int i0 = threadIdx.x * 8; i0 < prow; i0 += 256 * 8)
+        split_rows8<PREC>(x, out + (int64_t)j * ldo, rows, prow, stride, i0, sc);
 }
 
 // Transposed pieces: element (i, j) of X goes to out + i * ldo + j.  64 x 64
@@ -1247,6 +1279,26 @@ int gemm_pieces(int prec)
     case GEMM_F16X6: return 3;
     default: return 0;
     }
+}
+
+int gemm_split_exp(hipStream_t s, int prec, int rows, int cols, const float *X, int64_t ldx, int *col_exp,
+                   uint16_t *out, int64_t ldo, int64_t stride, int *launches)
+{
+    if (rows <= 0 || cols <= 0) return LATER_OK;
+    switch (prec) {
+    case GEMM_F16X3:
+        hipLaunchKernelGGL(split_exp_kernel<GEMM_F16X3>, dim3(cols), dim3(256), 0, s, rows, X, ldx, col_exp, out, ldo,
+                           stride);
+        break;
+    case GEMM_F16X6:
+        hipLaunchKernelGGL(split_exp_kernel<GEMM_F16X6>, dim3(cols), dim3(256), 0, s, rows, X, ldx, col_exp, out, ldo,
+                           stride);
+        break;
+    default: return LATER_ERR_OPTION;
+    }
+    HIP_OK(hipGetLastError());
+    if (launches) ++*launches;
+    return LATER_OK;
 }
 
 int gemm_split(hipStream_t s, int prec, int rows, int cols, const float *X, int64_t ldx, int a_exp,

@@ -201,29 +201,37 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     if (pre_proj && (rc = gemm_split(r.s, p, m, n1, A, lda, aexp, nullptr, (uint16_t *)q1h.p, q1h.ld, q1h.stride,
                                      false, &r.launches)))
         return rc;
-    // R12 = Q1^T A2   (QR/later_rgsqrf.cu:43-49)
-    if (scaled(p)) {
-        if ((rc = gemm_colexp(r.s, m, n2, A2, lda, r.exps, &r.launches))) return rc;
-        bexp = r.exps;
+    // R12 = Q1^T A2   (QR/later_rgsqrf.cu:43-49); A2's column exponents and
+    // pieces in one launch when it is staged.
+    if (scaled(p)) bexp = r.exps;
+    if (pre_proj && scaled(p)) {
+        if ((rc = gemm_split_exp(r.s, p, m, n2, A2, lda, r.exps, (uint16_t *)a2h.p, a2h.ld, a2h.stride,
+                                 &r.launches)))
+            return rc;
+    } else {
+        if (scaled(p) && (rc = gemm_colexp(r.s, m, n2, A2, lda, r.exps, &r.launches))) return rc;
+        if (pre_proj && (rc = gemm_split(r.s, p, m, n2, A2, lda, aexp, bexp, (uint16_t *)a2h.p, a2h.ld, a2h.stride,
+                                         false, &r.launches)))
+            return rc;
     }
-    if (pre_proj && (rc = gemm_split(r.s, p, m, n2, A2, lda, aexp, bexp, (uint16_t *)a2h.p, a2h.ld, a2h.stride,
-                                     false, &r.launches)))
-        return rc;
     rc = gemm_proj(r.s, p, m, n1, n2, A, lda, A2, lda, R12, ldr, chunk, bexp, aexp, r.gemm, r.gemm_n,
                    &r.launches, pre_proj ? &q1h : nullptr, pre_proj ? &a2h : nullptr);
     if (rc) return rc;
     // A2 -= Q1 R12    (QR/later_rgsqrf.cu:51-56)
-    if (scaled(p)) {
-        if ((rc = gemm_colexp(r.s, n1, n2, R12, ldr, r.exps, &r.launches))) return rc;
-    }
     if (pre) {
         const int pu = update_prec(p);
         if ((rc = gemm_split(r.s, pu, m, n1, A, lda, aexp, nullptr, (uint16_t *)q1t.p, q1t.ld, q1t.stride, true,
                              &r.launches)))
             return rc;
-        if ((rc = gemm_split(r.s, pu, n1, n2, R12, ldr, 0, bexp, (uint16_t *)r12h.p, r12h.ld, r12h.stride, false,
-                             &r.launches)))
-            return rc;
+        if (scaled(p))
+            rc = gemm_split_exp(r.s, pu, n1, n2, R12, ldr, r.exps, (uint16_t *)r12h.p, r12h.ld, r12h.stride,
+                                &r.launches);
+        else
+            rc = gemm_split(r.s, pu, n1, n2, R12, ldr, 0, bexp, (uint16_t *)r12h.p, r12h.ld, r12h.stride, false,
+                            &r.launches);
+        if (rc) return rc;
+    } else if (scaled(p)) {
+        if ((rc = gemm_colexp(r.s, n1, n2, R12, ldr, r.exps, &r.launches))) return rc;
     }
     rc = gemm_update(r.s, update_prec(p), m, n1, n2, A, lda, R12, ldr, A2, lda, f64, bexp, aexp, &r.launches,
                      pre ? &q1t : nullptr, pre ? &r12h : nullptr);
@@ -385,21 +393,21 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
                                1.0f);
             r.launches += 2;
             const int p2 = o.gemm_pass2, pu = update_prec(p2);
-            const int *bexp = nullptr;
-            if (scaled(p2)) {
-                rc = gemm_colexp(r.s, n, n, T, n, r.exps, &r.launches);
-                bexp = r.exps;
-            }
+            const int *bexp = scaled(p2) ? r.exps : nullptr;
             const int aexp = scaled(p2) ? kQExp : 0;
             const bool pre = r.pcs && n >= kPieceMinN && gemm_pieces(pu) > 0;
             Pieces rt{}, tt{};
-            if (!rc && pre) {
+            if (pre) {
                 const int64_t ld = piece_ld(n);
                 rt = {r.pcs, ld, ld * n};
                 tt = {r.pcs + gemm_pieces(pu) * ld * n, ld, ld * n};
                 rc = gemm_split(r.s, pu, n, n, R2, n, aexp, nullptr, (uint16_t *)rt.p, ld, ld * n, true, &r.launches);
-                if (!rc)
+                if (!rc && scaled(p2))
+                    rc = gemm_split_exp(r.s, pu, n, n, T, n, r.exps, (uint16_t *)tt.p, ld, ld * n, &r.launches);
+                else if (!rc)
                     rc = gemm_split(r.s, pu, n, n, T, n, 0, bexp, (uint16_t *)tt.p, ld, ld * n, false, &r.launches);
+            } else if (scaled(p2)) {
+                rc = gemm_colexp(r.s, n, n, T, n, r.exps, &r.launches);
             }
             if (!rc)
                 rc = gemm_nn(r.s, pu, n, n, n, 1.0f, R2, n, T, n, R, ldr, o.chunk > 0, bexp, aexp, true,
