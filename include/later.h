@@ -72,7 +72,9 @@ enum {
     LATER_PANEL_MGS = 0,    /* reference form: TSQR tree of block MGS, FP32 sums */
     LATER_PANEL_MGS64 = 1,  /* same tree, FP64 column norms and dot products */
     LATER_PANEL_CHOLQR2 = 2 /* Cholesky QR twice with FP64 Gram factors: three
-                               grid-wide launches, no sequential column loop */
+                               grid-wide launches, no sequential column loop;
+                               leaves of up to 64 columns (one pass in the
+                               second factorisation of LATER_REORTH_PASS) */
 };
 
 /* Re-orthogonalisation (bit flags; NODE|PASS = 3 combines both). */
@@ -97,7 +99,8 @@ typedef struct later_opts {
 void later_opts_default(later_opts *o);
 /* The accuracy-recovering mixed-precision form: FP16X3 first pass (FP64
  * folds every 256 rows), FP16X6 second-pass projections (folds every 32
- * rows), Cholesky QR twice at the leaves, second full pass. */
+ * rows), Cholesky QR leaves of up to 64 columns (twice in the first pass,
+ * once in the second), second full pass. */
 void later_opts_mixed(later_opts *o);
 /* The plain-FP32 form: FP32 MFMA everywhere, reference panel, one pass. */
 void later_opts_fp32(later_opts *o);

@@ -287,10 +287,10 @@ LATER_API void later_opts_fp32(later_opts *o)
 
 // Mixed form: FP16 hi/lo products with FP64 folds every 256 rows in the
 // first pass; three-piece products folded every 32 rows in the second pass,
-// whose projections set the orthogonality floor; Cholesky QR twice at the
-// leaves.  At 16384^2 (N(0,1)) this gives ||I - Q^T Q||_F = 8.8e-6 and
-// ||A - QR||_F / ||A||_F = 3.4e-7; the same with two-piece second-pass
-// products gives 1.3e-5.
+// whose projections set the orthogonality floor; Cholesky QR leaves of up to
+// 64 columns (twice in the first pass, once in the second).  At 16384^2
+// (N(0,1)) this gives ||I - Q^T Q||_F = 8.6e-6 and ||A - QR||_F / ||A||_F =
+// 3.4e-7; the same with two-piece second-pass products gives 1.3e-5.
 LATER_API void later_opts_mixed(later_opts *o)
 {
     later_opts_default(o);
