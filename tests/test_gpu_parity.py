@@ -393,11 +393,13 @@ def _graded(m, n, decades, seed=0):
     return np.asfortranarray((U * s) @ V.T, dtype=np.float32)
 
 
-@pytest.mark.parametrize("m,n", [(4096, 64), (300, 50), (100000, 64), (1000, 33), (64, 64)])
+@pytest.mark.parametrize("m,n", [(4096, 64), (300, 50), (100000, 64), (1000, 33), (64, 64), (500, 65),
+                                 (2000, 97)])
 def test_cholqr2_wide_leaf(later, oracle, m, n):
-    """One Cholesky QR leaf of 33..64 columns (the blocked 64 x 64 factor and
-    inverse) in the mixed form, kappa = 1e6: Householder-level orthogonality,
-    backward error and R against LAPACK in FP64."""
+    """Cholesky QR leaves of 33..64 columns (the blocked 64 x 64 factor and
+    inverse) in the mixed form, alone or beside a narrow leaf (65 = 64 + 1,
+    97 = 64 + 33), kappa = 1e6: Householder-level orthogonality, backward
+    error and R against LAPACK in FP64."""
     A = _graded(m, n, 6)
     Q, R, info = _run(later, A, "mixed")
     Q64, R64 = oracle.lapack_qr64(A)
