@@ -359,7 +359,7 @@ template <int W> __global__ __launch_bounds__(256) void chol_reduce_kernel(const
     const int per = (ngroups + 7) / 8, g0 = q * per, g1 = min(ngroups, g0 + per);
     double s = 0.0;
     if (e < Shape<W>::P) {
-        constexpr int kB = 16;
+        constexpr int kB = 32;  // one round of loads for up to 256 partials
         for (int g = g0; g < g1; g += kB) {
             double v[kB];
 #pragma unroll
