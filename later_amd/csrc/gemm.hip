@@ -1055,12 +1055,23 @@ void launch_tile(hipStream_t s, const GemmArgs &g, int nsplit, bool small)
             launch_one<PREC, 64, 64, true, CH64, true>(s, g, g64);
         } else {
             // The split forms run eight waves (two per SIMD): 256-row tiles
-            // for the two-piece form without FP64 folds, 128-row tiles where
-            // the folds' FP64 accumulators take the registers.
-            constexpr int BM = (PREC == GEMM_F16X3 && !CH64) ? 256 : 128;
+            // for the two-piece form without FP64 folds (unless that leaves
+            // fewer tiles than CUs: small), 128-row tiles where the folds'
+            // FP64 accumulators take the registers.
             constexpr int WGM = (PREC == GEMM_F16X6 || PREC == GEMM_F16X3) ? 4 : 2;
-            const dim3 grid((unsigned)ceil_div64(g.M, BM), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
-            hipLaunchKernelGGL((gemm_lds_kernel<PREC, CH64, BM, WGM>), grid, dim3(128 * WGM), 0, s, g);
+            bool big = false;
+            if constexpr (PREC == GEMM_F16X3 && !CH64) {
+                big = !small;
+                if (big) {
+                    const dim3 grid((unsigned)ceil_div64(g.M, 256), (unsigned)ceil_div64(g.N, 128),
+                                    (unsigned)nsplit);
+                    hipLaunchKernelGGL((gemm_lds_kernel<PREC, CH64, 256, WGM>), grid, dim3(128 * WGM), 0, s, g);
+                }
+            }
+            if (!big) {
+                const dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
+                hipLaunchKernelGGL((gemm_lds_kernel<PREC, CH64, 128, WGM>), grid, dim3(128 * WGM), 0, s, g);
+            }
         }
     } else {
         const dim3 g64((unsigned)ceil_div64(g.M, 64), (unsigned)ceil_div64(g.N, 64), (unsigned)nsplit);
@@ -1251,7 +1262,9 @@ int gemm_nn(hipStream_t s, int prec, int M, int N, int K, float alpha, const flo
     g.alpha = alpha;
     g.tri = tri ? 1 : 0;
     set_pieces(g, ah, bh);
-    const bool small = (N <= 64);
+    // Thin products, and products with fewer 128 x 128 tiles than the chip's
+    // 256 CUs, run the smaller tiles.
+    const bool small = (N <= 64) || ceil_div64(M, 128) * ceil_div64(N, 128) < 256;
     const int tok = prof_start(s);
     int rc = dispatch<false, false>(s, prec, g, 1, small);
     const bool x3 = prec == GEMM_F16X3 && !small && g.Ah && g.Bh;
