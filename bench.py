@@ -16,8 +16,10 @@ is reported with FP64 accumulation: ||I - Q^T Q||_F and ||A - QR||_F/||A||_F.
 roofline: the dominant kernel (node GEMMs) timed live with HIP events on the
 launch stream; achieved = algorithmic flops 2 m n1 n2 per projection / update
 launch summed over one step, divided by the summed kernel time.
-cpu_baseline: the oracle's C restatement of the reference recursion (FP32,
-OpenMP) on a bounded square sample, TFLOP/s by the same formula.
+cpu_baseline: host LAPACK sgeqrf + sorgqr (scipy's OpenBLAS, threads = the
+process's cores) on a bounded square sample and on the 8192 x 1024 workload,
+beside the oracle's C restatement of the FP32 recursion (OpenMP), TFLOP/s by
+the same formula.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 16384]
 """
@@ -284,18 +286,55 @@ def read_traffic(k):
     return float(mt.group(1)) if mt else None
 
 
-def cpu_baseline(n_sample: int):
-    from oracle import oracle as O
+def _lapack_qr_seconds(A):
+    """Host LAPACK sgeqrf + sorgqr (explicit Q, as rgsqrf returns) on a copy of A."""
+    from scipy.linalg import lapack
     import numpy as np
-    A = O.gen_normal(n_sample, n_sample, 0, 0)
+    X = np.array(A, dtype=np.float32, order="F", copy=True)
     t = time.perf_counter()
-    O.rgsqrf(A, "f32")
+    qr, tau, _, info = lapack.sgeqrf(X, overwrite_a=1)
+    q, _, info2 = lapack.sorgqr(qr, tau, overwrite_a=1)
     dt = time.perf_counter() - t
-    flops = 2.0 * n_sample * n_sample * (n_sample - n_sample / 3.0)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"oracle C restatement of the FP32 recursion on m=n={n_sample} N(0,1) seed 0, "
-                      f"{dt:.1f} s, OpenMP threads={cores}"}
+    if info or info2:
+        raise RuntimeError(f"LAPACK status {info} {info2}")
+    return dt, q
+
+
+def cpu_baseline(n_sample: int, n_port: int):
+    """Host-core baselines (reported only): LAPACK sgeqrf + sorgqr through
+    scipy's OpenBLAS with threads = the process's cores, on (1) the 8192 x 1024
+    uniform workload at full size (one warm-up, median of 5) and (2) a square
+    N(0,1) sample of the headline workload (one run); and the oracle's C
+    restatement of the FP32 recursion (OpenMP) on a smaller square sample."""
+    from oracle import oracle as O
+    import scipy.linalg.lapack  # noqa: F401  (loads its BLAS before the thread limit is set)
+    from threadpoolctl import threadpool_info, threadpool_limits
+    cores = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        cores = min(cores, int(omp))
+    ref = lambda m, n: 2.0 * n * n * (m - n / 3.0)
+    with threadpool_limits(limits=cores, user_api="blas"):
+        blas = [f"{i.get('internal_api')} {i.get('version')} threads={i.get('num_threads')}"
+                for i in threadpool_info() if i.get("user_api") == "blas"]
+        A1 = O.gen_uniform(8192, 1024, 0, 0)
+        _lapack_qr_seconds(A1)
+        c1 = sorted(_lapack_qr_seconds(A1)[0] for _ in range(5))[2]
+        As = O.gen_normal(n_sample, n_sample, 0, 0)
+        dts, Q = _lapack_qr_seconds(As)
+        orth = O.orth(Q)
+    Ap = O.gen_normal(n_port, n_port, 0, 0)
+    t = time.perf_counter()
+    O.rgsqrf(Ap, "f32")
+    dtp = time.perf_counter() - t
+    return {"value": ref(n_sample, n_sample) / dts / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+            "what": "host LAPACK sgeqrf + sorgqr (scipy's OpenBLAS), explicit Q; rate by 2 n^2 (m - n/3)",
+            "sample": f"square m=n={n_sample} N(0,1) seed 0, one run after a warm-up: {dts:.2f} s, "
+                      f"||I-Q^TQ||_F={orth:.2e}; BLAS: {'; '.join(blas)}",
+            "c1_8192x1024": {"seconds_median_of_5": c1, "tflops_ref": ref(8192, 1024) / c1 / 1e12},
+            "restatement": {"tflops_ref": ref(n_port, n_port) / dtp / 1e12, "seconds": dtp,
+                            "sample": f"oracle C restatement of the FP32 recursion, m=n={n_port} N(0,1), "
+                                      f"OpenMP threads={omp or os.cpu_count()}"}}
 
 
 def main():
@@ -305,7 +344,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--m", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=6144)
+    ap.add_argument("--cpu-sample", type=int, default=6144, help="square sample of the LAPACK baseline")
+    ap.add_argument("--cpu-port-sample", type=int, default=4096, help="square sample of the restatement")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
@@ -401,7 +441,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline sample n={args.cpu_sample}")
-        cpu = cpu_baseline(args.cpu_sample)
+        cpu = cpu_baseline(args.cpu_sample, args.cpu_port_sample)
 
     out = {
         "metric": "QR TFLOP/s 16384^2 mixed vs plain FP32 at ||I-Q^TQ||<=1e-5; LS-solve p50/p99",

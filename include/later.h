@@ -115,7 +115,12 @@ int later_rgsqrf_workspace(int m, int n, const later_opts *opts, int64_t *lwork,
 
 /* Reference entry point, same argument list as include/LATER.h:39 (hwork is
  * the reference's __half*).  Runs the default options.  Returns nothing, as
- * the reference; a shape or workspace error leaves A and R untouched. */
+ * the reference; a shape or workspace error leaves A and R untouched.  The
+ * reference's own caller sizes (lwork = m/256*32*n floats, lhwork = m*n
+ * halves, test/test_qr.cu:73-76) always suffice: what does not fit in `work`
+ * goes to `hwork` (see later_read_status).  R may alias `work`, as the
+ * reference's polar-decomposition caller passes it (EVD/later_qdwh_polar.cu:79):
+ * the whole workspace then lives in `hwork`. */
 void later_rgsqrf(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, int ldr,
                   float *work, int lwork, uint16_t *hwork, int lhwork);
 
@@ -167,7 +172,10 @@ int later_batch_plan_destroy(void *plan);
 /* Synchronises ctxt.stream and reads the device status word that the last
  * later_rgsqrf_ex / later_caqr_panel call on this `work` left: bit 0 set
  * means a zero or non-finite column norm (LATER_ERR_BREAKDOWN); the
- * affected Q column is zero and its R diagonal entry 0. */
+ * affected Q column is zero and its R diagonal entry 0.  (A non-finite input
+ * sets the bit too; Q and R are then unspecified.)  The word is work[0]
+ * whenever lwork >= 64; with a smaller `work` (the reference's caller sizes
+ * give lwork = 0 for m < 256) it is the first word of hwork: pass hwork. */
 int later_read_status(laterCtxt ctxt, const float *work, int *device_status);
 
 /* Panel of the reference alone: mgs_caqr_panel_256x32 (QR/panel.cu:65-134)
@@ -180,13 +188,22 @@ int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, float *R, 
 /* The two node GEMMs alone (QR/later_rgsqrf.cu:45-49 and :52-56):
  *   later_gemm_proj:   C (n1 x n2, ldc)  = Q^T B,   Q m x n1, B m x n2
  *   later_gemm_update: B (m x n2, ldb)  -= Q C
- * prec is LATER_GEMM_*; chunk as in later_opts. */
+ * prec is LATER_GEMM_*; chunk as in later_opts.  hwork (optional, 16-bit,
+ * lhwork elements): when at least the query's *lhwork, the operands are
+ * first split into FP16/BF16 pieces there (the reference's s2h staging,
+ * util/util.cu:24-32) and the products stream the pieces -- the kernels the
+ * recursion's wide nodes run; NULL converts in the kernel.
+ * later_gemm_workspace: *lwork FP32 elements of `work` (column exponents and
+ * split-K partials) and *lhwork 16-bit elements of the pieces, for the
+ * projection (update = 0) or the update (update = 1). */
+int later_gemm_workspace(int m, int n1, int n2, int prec, int chunk, int update, int64_t *lwork,
+                         int64_t *lhwork);
 int later_gemm_proj(laterCtxt ctxt, int m, int n1, int n2, const float *Q, int ldq,
                     const float *B, int ldb, float *C, int ldc, int prec, int chunk,
-                    float *work, int64_t lwork);
+                    float *work, int64_t lwork, uint16_t *hwork, int64_t lhwork);
 int later_gemm_update(laterCtxt ctxt, int m, int n1, int n2, const float *Q, int ldq,
                       const float *C, int ldc, float *B, int ldb, int prec, float *work,
-                      int64_t lwork);
+                      int64_t lwork, uint16_t *hwork, int64_t lhwork);
 
 /* Least squares (the LS-solve of the serving path): triangular solve
  * B (n x k, ldb) <- R^{-1} B with R upper triangular (n <= 8192), FP64 sums;

@@ -71,8 +71,9 @@ _SIGNATURES = {
                              ctypes.POINTER(later_opts), ctypes.POINTER(later_info)]),
     "later_read_status": (_i, [laterCtxt, _vp, ctypes.POINTER(_i)]),
     "later_caqr_panel": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _vp, _i64, _i]),
-    "later_gemm_proj": (_i, [laterCtxt, _i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _vp, _i64]),
-    "later_gemm_update": (_i, [laterCtxt, _i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _i64]),
+    "later_gemm_workspace": (_i, [_i, _i, _i, _i, _i, _i, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "later_gemm_proj": (_i, [laterCtxt, _i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _vp, _i64, _vp, _i64]),
+    "later_gemm_update": (_i, [laterCtxt, _i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _i64, _vp, _i64]),
     "later_generate_uniform": (_i, [laterCtxt, _i64, _i64, _vp, _i64, _u32, _u32]),
     "later_generate_normal": (_i, [laterCtxt, _i64, _i64, _vp, _i64, _u32, _u32]),
     "later_check_orth": (_i, [laterCtxt, _i, _i, _vp, _i, ctypes.POINTER(_d)]),
@@ -342,27 +343,46 @@ def caqr_panel(A, R=None, fp64=False, work=None, stream=None):
     return R
 
 
-def gemm_proj(Q, B, C, prec: int, chunk: int = 0, work=None, stream=None):
+def gemm_workspace(m: int, n1: int, n2: int, prec: int, chunk: int = 0, update: bool = False) -> tuple[int, int]:
+    """(lwork, lhwork) of one component GEMM: FP32 work and 16-bit pieces."""
+    lw, lh = _i64(), _i64()
+    _check(lib().later_gemm_workspace(m, n1, n2, prec, chunk, int(update), ctypes.byref(lw), ctypes.byref(lh)),
+           "later_gemm_workspace")
+    return int(lw.value), int(lh.value)
+
+
+def _gemm_bufs(device, lw, lh, pieces):
     torch = _torch()
+    work = torch.empty(max(lw, 1), device=device, dtype=torch.float32)
+    hwork = torch.empty(lh, device=device, dtype=torch.int16) if (pieces and lh > 0) else None
+    return work, hwork
+
+
+def gemm_proj(Q, B, C, prec: int, chunk: int = 0, work=None, stream=None, pieces: bool = False):
+    """C = Q^T B.  pieces=True stages the operands as pre-split 16-bit pieces
+    (the streaming kernels of the recursion's wide nodes)."""
     m, n1 = Q.shape
     n2 = B.shape[1]
-    if work is None:
-        work = torch.empty(64 + n2 + 2 * 64 * n1 * n2 + (m // 256 + 2) * n1 * n2 * 2, device=Q.device,
-                           dtype=torch.float32)
+    lw, lh = gemm_workspace(m, n1, n2, prec, chunk, False)
+    w, hwork = _gemm_bufs(Q.device, lw, lh, pieces)
+    work = w if work is None else work
+    hp, hn = _hw(hwork)
     _check(lib().later_gemm_proj(ctxt(stream), m, n1, n2, Q.data_ptr(), _ld(Q), B.data_ptr(), _ld(B),
-                                 C.data_ptr(), _ld(C), prec, chunk, work.data_ptr(), work.numel()),
+                                 C.data_ptr(), _ld(C), prec, chunk, work.data_ptr(), work.numel(), hp, hn),
            "later_gemm_proj")
     return C
 
 
-def gemm_update(Q, C, B, prec: int, work=None, stream=None):
-    torch = _torch()
+def gemm_update(Q, C, B, prec: int, work=None, stream=None, pieces: bool = False):
+    """B -= Q C (pieces as for gemm_proj)."""
     m, n1 = Q.shape
     n2 = C.shape[1]
-    if work is None:
-        work = torch.empty(n2 + 64, device=Q.device, dtype=torch.float32)
+    lw, lh = gemm_workspace(m, n1, n2, prec, 0, True)
+    w, hwork = _gemm_bufs(Q.device, lw, lh, pieces)
+    work = w if work is None else work
+    hp, hn = _hw(hwork)
     _check(lib().later_gemm_update(ctxt(stream), m, n1, n2, Q.data_ptr(), _ld(Q), C.data_ptr(), _ld(C),
-                                   B.data_ptr(), _ld(B), prec, work.data_ptr(), work.numel()),
+                                   B.data_ptr(), _ld(B), prec, work.data_ptr(), work.numel(), hp, hn),
            "later_gemm_update")
     return B
 

@@ -46,7 +46,29 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_driver(verbose)
     return LIB
+
+
+DRIVER_SRC = os.path.join(HERE, "test_qr.c")
+DRIVER = os.path.join(HERE, "test_qr")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The plain-C command-line driver (test_qr algo m n [-check]), compiled
+    by gcc against include/later.h and linked to the library."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = ["gcc", "-O2", "-std=c11", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{rocm}/include", DRIVER_SRC,
+           "-o", DRIVER + ".tmp", f"-L{HERE}", "-llater_amd", f"-L{rocm}/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{rocm}/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("driver build failed")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -1174,7 +1174,11 @@ static void proj_plan(int m, int n1, int n2, int chunk, bool *small, int *nsplit
     const int64_t maxs = m / 256 > 1 ? m / 256 : 1;
     if (ns > maxs) ns = maxs;
     const int64_t per = (int64_t)n1 * n2 * (chunk > 0 ? 2 : 1);
-    const int64_t budget = (int64_t)m * (n1 + n2) / (thin ? 8 : 2) / per;
+    // The 64 Ki floats held back keep the partials, with the rest of the
+    // factorisation's FP32 workspace, inside m n / 2 floats: the reference's
+    // callers size the 16-bit arena at m n halves (test/test_qr.cu:76,
+    // EVD/later_qdwh_polar.cu:75), which then holds everything.
+    const int64_t budget = ((int64_t)m * (n1 + n2) / (thin ? 8 : 2) - 65536) / per;
     if (ns > budget) ns = budget;
     if (ns < 1) ns = 1;
     int64_t kp = ceil_div64(m, ns);

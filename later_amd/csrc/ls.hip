@@ -82,7 +82,9 @@ LATER_API int later_ls_workspace(int m, int n, int k, const later_opts *opts, in
     int64_t lw = 0, lh = 0;
     int rc = later_rgsqrf_workspace(m, n, opts, &lw, &lh);
     if (rc) return rc;
-    const int64_t proj = ((int64_t)k + 63) / 64 * 64 + gemm_proj_work_floats(m, n, k, 256);
+    // later_gels reuses `work` after the QR: the status header (64 floats),
+    // the exponents of B's columns, then the split-K partials of Q^T B.
+    const int64_t proj = 64 + ((int64_t)k + 63) / 64 * 64 + gemm_proj_work_floats(m, n, k, 256);
     if (lwork) *lwork = (lw > proj ? lw : proj);
     return LATER_OK;
 }

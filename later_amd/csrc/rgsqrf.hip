@@ -327,16 +327,23 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
              (o.chunk2 % 32) != 0)
         rc = LATER_ERR_OPTION;
     Layout L = layout(m > 0 ? m : 1, n > 0 ? n : 1, o);
-    // Everything lives in `work` when it is large enough; otherwise the
-    // split-K partials may use the 16-bit arena `hwork` (the reference sizes
-    // it at m*n halves, test/test_qr.cu:76).
-    const bool partial_in_h = work && lwork < L.total && lwork >= L.total - L.gemm && hwork &&
-                              lhwork / 2 >= L.gemm;
-    if (!rc && (!work || (lwork < L.total && !partial_in_h))) rc = LATER_ERR_WORKSPACE;
+    // Everything lives in `work` when it is large enough.  Otherwise the
+    // 16-bit arena `hwork` (which the reference's caller sizes at m*n halves,
+    // test/test_qr.cu:76, next to lwork = m/256*32*n floats, :73) takes the
+    // split-K partials, or, when `work` cannot even hold the rest, the whole
+    // FP32 layout.  The status word stays at work[0] whenever `work` has room
+    // for the header (lwork is 0 for m < 256 there: then it is hwork's).
+    const bool h_aligned = hwork && (((uintptr_t)hwork) & 255) == 0;
+    const bool work_fits = work && lwork >= L.total;
+    const bool hdr_in_work = work && lwork >= L.header;
+    const bool partial_in_h = !work_fits && work && lwork >= L.total - L.gemm && h_aligned && lhwork / 2 >= L.gemm;
+    const bool all_in_h = !work_fits && !partial_in_h && h_aligned &&
+                          lhwork / 2 >= L.total - (hdr_in_work ? L.header : 0);
+    if (!rc && !work_fits && !partial_in_h && !all_in_h) rc = LATER_ERR_WORKSPACE;
     // The 16-bit arena holds the pre-split GEMM operands when it is large
-    // enough (and not used for partials); otherwise the GEMMs convert their
-    // operands in the kernel, with the same result bits.
-    const bool with_pieces = !partial_in_h && hwork && L.hpieces > 0 && lhwork >= L.hpieces;
+    // enough (and not used for FP32 workspace); otherwise the GEMMs convert
+    // their operands in the kernel, with the same result bits.
+    const bool with_pieces = work_fits && hwork && L.hpieces > 0 && lhwork >= L.hpieces;
     if (!rc) {
         const int64_t abytes = ((int64_t)(n - 1) * lda + m) * 4, rbytes = ((int64_t)(n - 1) * ldr + n) * 4;
         const int64_t wbytes = lwork * 4, hbytes = lhwork * 2;
@@ -352,8 +359,13 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     Run r;
     r.s = ctx_stream(ctxt);
     r.o = o;
-    float *p = work;
-    r.status = (int *)p; p += L.header;
+    float *p = all_in_h ? (float *)hwork : work;
+    if (all_in_h && hdr_in_work) {
+        r.status = (int *)work;
+    } else {
+        r.status = (int *)p;
+        p += L.header;
+    }
     r.exps = (int *)p; p += L.exps;
     r.panel = p; r.panel_n = L.panel; p += L.panel;
     if (partial_in_h) {
@@ -428,7 +440,17 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
 LATER_API void later_rgsqrf(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, int ldr, float *work,
                             int lwork, uint16_t *hwork, int lhwork)
 {
-    later_rgsqrf_ex(ctxt, m, n, A, lda, R, ldr, work, lwork, hwork, lhwork, nullptr, nullptr);
+    const int rc = later_rgsqrf_ex(ctxt, m, n, A, lda, R, ldr, work, lwork, hwork, lhwork, nullptr, nullptr);
+    // The reference's polar-decomposition caller passes R == work and reads
+    // only Q afterwards (EVD/later_qdwh_polar.cu:79-84).  The status form
+    // rejects that aliasing; here the factorisation then keeps its whole
+    // FP32 workspace in the 16-bit arena (the caller's 2n*n halves hold it),
+    // so that Q and R are both written.
+    if (rc == LATER_ERR_ALIAS && hwork && R && work && lwork >= 0) {
+        const int64_t rbytes = ((int64_t)(n - 1) * ldr + n) * 4;
+        if (overlap(R, rbytes, work, (int64_t)lwork * 4 + rbytes))
+            later_rgsqrf_ex(ctxt, m, n, A, lda, R, ldr, nullptr, 0, hwork, lhwork, nullptr, nullptr);
+    }
 }
 
 LATER_API int later_read_status(laterCtxt ctxt, const float *work, int *device_status)
@@ -456,40 +478,92 @@ LATER_API int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, 
                       (int *)work, nullptr);
 }
 
+// 16-bit elements of the pre-split operand pieces of one component GEMM, in
+// the layout the recursion's nodes use: the projection stages Q and B
+// column-major (ld piece_ld(m)); the update stages Q transposed (ld
+// piece_ld(n1), m rows) and C column-major (ld piece_ld(n1)).
+static int64_t component_pieces(int m, int n1, int n2, int prec, bool update)
+{
+    const int np = gemm_pieces(prec);
+    if (np == 0) return 0;
+    if (!update) return (int64_t)np * piece_ld(m) * ((int64_t)n1 + n2);
+    return (int64_t)np * piece_ld(n1) * ((int64_t)m + n2);
+}
+
+LATER_API int later_gemm_workspace(int m, int n1, int n2, int prec, int chunk, int update, int64_t *lwork,
+                                   int64_t *lhwork)
+{
+    if (m < 1 || n1 < 1 || n2 < 1) return LATER_ERR_SHAPE;
+    if (!prec_ok(prec) || chunk < 0 || chunk % 32) return LATER_ERR_OPTION;
+    const int64_t exps = ((int64_t)n2 + 63) / 64 * 64;
+    if (lwork) *lwork = update ? exps : exps + gemm_proj_work_floats(m, n1, n2, chunk);
+    if (lhwork) *lhwork = component_pieces(m, n1, n2, prec, update != 0);
+    return LATER_OK;
+}
+
 LATER_API int later_gemm_proj(laterCtxt ctxt, int m, int n1, int n2, const float *Q, int ldq, const float *B,
-                              int ldb, float *C, int ldc, int prec, int chunk, float *work, int64_t lwork)
+                              int ldb, float *C, int ldc, int prec, int chunk, float *work, int64_t lwork,
+                              uint16_t *hwork, int64_t lhwork)
 {
     if (m < 1 || n1 < 1 || n2 < 1 || ldq < m || ldb < m || ldc < n1 || !Q || !B || !C) return LATER_ERR_SHAPE;
     if (!prec_ok(prec) || chunk < 0 || chunk % 32) return LATER_ERR_OPTION;
     const int64_t exps = ((int64_t)n2 + 63) / 64 * 64;
     const int64_t need = exps + gemm_proj_work_floats(m, n1, n2, chunk);
     if (lwork < need || (!work && need > 0)) return LATER_ERR_WORKSPACE;
+    const int64_t hneed = component_pieces(m, n1, n2, prec, false);
+    const bool pre = hwork && hneed > 0 && lhwork >= hneed && (((uintptr_t)hwork) & 15) == 0;
     hipStream_t s = ctx_stream(ctxt);
-    const int *bexp = nullptr;
-    if (scaled(prec)) {
-        int rc = gemm_colexp(s, m, n2, B, ldb, (int *)work, nullptr);
-        if (rc) return rc;
-        bexp = (const int *)work;
+    const int aexp = scaled(prec) ? kQExp : 0;
+    const int *bexp = scaled(prec) ? (const int *)work : nullptr;
+    Pieces qh{}, bh{};
+    int rc = LATER_OK;
+    if (pre) {
+        const int64_t ld = piece_ld(m), np = gemm_pieces(prec);
+        qh = {hwork, ld, ld * n1};
+        bh = {hwork + np * ld * n1, ld, ld * n2};
+        rc = gemm_split(s, prec, m, n1, Q, ldq, aexp, nullptr, hwork, qh.ld, qh.stride, false, nullptr);
+        if (!rc && scaled(prec))
+            rc = gemm_split_exp(s, prec, m, n2, B, ldb, (int *)work, (uint16_t *)bh.p, bh.ld, bh.stride, nullptr);
+        else if (!rc)
+            rc = gemm_split(s, prec, m, n2, B, ldb, 0, nullptr, (uint16_t *)bh.p, bh.ld, bh.stride, false, nullptr);
+    } else if (scaled(prec)) {
+        rc = gemm_colexp(s, m, n2, B, ldb, (int *)work, nullptr);
     }
-    return gemm_proj(s, prec, m, n1, n2, Q, ldq, B, ldb, C, ldc, chunk, bexp, scaled(prec) ? kQExp : 0,
-                     work + exps, lwork - exps, nullptr);
+    if (rc) return rc;
+    return gemm_proj(s, prec, m, n1, n2, Q, ldq, B, ldb, C, ldc, chunk, bexp, aexp, work + exps, lwork - exps,
+                     nullptr, pre ? &qh : nullptr, pre ? &bh : nullptr);
 }
 
 LATER_API int later_gemm_update(laterCtxt ctxt, int m, int n1, int n2, const float *Q, int ldq, const float *C,
-                                int ldc, float *B, int ldb, int prec, float *work, int64_t lwork)
+                                int ldc, float *B, int ldb, int prec, float *work, int64_t lwork, uint16_t *hwork,
+                                int64_t lhwork)
 {
     if (m < 1 || n1 < 1 || n2 < 1 || ldq < m || ldb < m || ldc < n1 || !Q || !B || !C) return LATER_ERR_SHAPE;
     if (!prec_ok(prec)) return LATER_ERR_OPTION;
     const int64_t exps = ((int64_t)n2 + 63) / 64 * 64;
     if (scaled(prec) && (lwork < exps || !work)) return LATER_ERR_WORKSPACE;
+    const int64_t hneed = component_pieces(m, n1, n2, prec, true);
+    const bool pre = hwork && hneed > 0 && lhwork >= hneed && (((uintptr_t)hwork) & 15) == 0;
     hipStream_t s = ctx_stream(ctxt);
-    const int *bexp = nullptr;
-    if (scaled(prec)) {
-        int rc = gemm_colexp(s, n1, n2, C, ldc, (int *)work, nullptr);
-        if (rc) return rc;
-        bexp = (const int *)work;
+    const int aexp = scaled(prec) ? kQExp : 0;
+    const int *bexp = scaled(prec) ? (const int *)work : nullptr;
+    Pieces qt{}, ch{};
+    int rc = LATER_OK;
+    if (pre) {
+        const int64_t ld = piece_ld(n1), np = gemm_pieces(prec);
+        qt = {hwork, ld, ld * m};
+        ch = {hwork + np * ld * m, ld, ld * n2};
+        rc = gemm_split(s, prec, m, n1, Q, ldq, aexp, nullptr, hwork, qt.ld, qt.stride, true, nullptr);
+        if (!rc && scaled(prec))
+            rc = gemm_split_exp(s, prec, n1, n2, C, ldc, (int *)work, (uint16_t *)ch.p, ch.ld, ch.stride, nullptr);
+        else if (!rc)
+            rc = gemm_split(s, prec, n1, n2, C, ldc, 0, nullptr, (uint16_t *)ch.p, ch.ld, ch.stride, false, nullptr);
+    } else if (scaled(prec)) {
+        rc = gemm_colexp(s, n1, n2, C, ldc, (int *)work, nullptr);
     }
-    return gemm_update(s, prec, m, n1, n2, Q, ldq, C, ldc, B, ldb, false, bexp, scaled(prec) ? kQExp : 0, nullptr);
+    if (rc) return rc;
+    return gemm_update(s, prec, m, n1, n2, Q, ldq, C, ldc, B, ldb, false, bexp, aexp, nullptr, pre ? &qt : nullptr,
+                       pre ? &ch : nullptr);
 }
 
 LATER_API const char *later_version(void) { return "later_amd 0.1 (gfx950)"; }

@@ -439,7 +439,14 @@ LATER_API int later_server_submit(void *srv, int m, int n, int k, const float *A
         ++S->rejected;
         return LATER_ERR_BUDGET;
     }
-    if (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess) return LATER_ERR_DEVICE;
+    if (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess) {
+        // Recorded as failed (never admitted), so it can still be released.
+        r->ev = nullptr;
+        r->state = FAILED;
+        r->status = LATER_ERR_DEVICE;
+        r->t_done = r->t_submit;
+        return LATER_ERR_DEVICE;
+    }
     S->waiting.push_back(r);
     progress(*S);
     return LATER_OK;
@@ -494,13 +501,12 @@ LATER_API int later_server_fetch(void *srv, uint64_t id, float *X, int ldx, floa
 {
     Server *S = (Server *)srv;
     if (!S) return LATER_ERR_SHAPE;
-    Req *r;
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        auto it = S->reqs.find(id);
-        if (it == S->reqs.end() || it->second->state != DONE) return LATER_ERR_SHAPE;
-        r = it->second;
-    }
+    // Held through the copies: a concurrent later_server_release cannot free
+    // the request (and its pool memory) under them.
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end() || it->second->state != DONE) return LATER_ERR_SHAPE;
+    const Req *r = it->second;
     if (X && ldx < r->n) return LATER_ERR_SHAPE;
     if (R && ldr < r->n) return LATER_ERR_SHAPE;
     if (Q && ldq < r->m) return LATER_ERR_SHAPE;
