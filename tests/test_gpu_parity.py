@@ -315,9 +315,20 @@ def test_breakdown_sets_status(later, oracle):
 def test_workspace_too_small_is_an_error(later):
     import torch
     A = later.generate_uniform(4096, 512)
-    with pytest.raises(later.LaterError) as e:
-        later.rgsqrf(A, o=later.opts("mixed"), work=torch.empty(10, device="cuda"))
-    assert e.value.code == later.ERR_WORKSPACE
+    A0 = A.clone()
+    for hw in (None, torch.empty(4096, device="cuda", dtype=torch.int16)):
+        with pytest.raises(later.LaterError) as e:
+            later.rgsqrf(A, o=later.opts("mixed"), work=torch.empty(10, device="cuda"), hwork=hw)
+        assert e.value.code == later.ERR_WORKSPACE
+    torch.cuda.synchronize()
+    assert torch.equal(A, A0)  # nothing was enqueued
+    # a 16-bit arena large enough for the whole FP32 layout takes it instead
+    lw = later.workspace(4096, 512, later.opts("mixed")).numel()
+    hw = torch.empty(2 * lw + 256, device="cuda", dtype=torch.int16)
+    R = later.rgsqrf(A, o=later.opts("mixed"), work=torch.empty(10, device="cuda"), hwork=hw)
+    Rf = later.rgsqrf(A0, o=later.opts("mixed"), hwork=None)
+    torch.cuda.synchronize()
+    assert torch.equal(R, Rf) and torch.equal(A, A0)
 
 
 # ---------------------------------------------------------------- -check
