@@ -138,6 +138,23 @@ int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, i
                     float *work, int64_t lwork, uint16_t *hwork, int64_t lhwork,
                     const later_opts *opts, later_info *info);
 
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  later_rgsqrf_ex with its FP32 workspace in a file-scope
+ * device array of LATER_NOALLOC_SCRATCH_FLOATS floats (160 MiB; each GPU the
+ * entry point runs on holds its own copy until the process ends) and no
+ * 16-bit arena (in-kernel operand conversion, the same result bits).  The
+ * caller still passes A and R and has the stream's device current.  Shapes
+ * and options whose later_rgsqrf_workspace exceeds the array return
+ * LATER_ERR_WORKSPACE: every reference- and FP32-form call up to 16384^2 and
+ * the tall-skinny 1048576 x 128 and 262144 x 256 fit, the mixed form up to
+ * 4096^2.  Nothing is allocated or freed after the first call on a device
+ * (which looks up the array's device address).  All calls on one GPU share
+ * the array: calls that overlap in time on one GPU race. */
+#define LATER_NOALLOC_SCRATCH_FLOATS (40 * 1024 * 1024)
+int later_rgsqrf_noalloc(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, int ldr,
+                         const later_opts *opts, later_info *info);
+
 /* Graph-captured plans (no counterpart in the reference, which launches
  * eagerly): later_plan_create records every kernel of one later_rgsqrf_ex
  * call on these exact buffers and options into a HIP graph (it enqueues

@@ -69,6 +69,8 @@ _SIGNATURES = {
     "later_rgsqrf": (None, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i]),
     "later_rgsqrf_ex": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _vp, _i64, _vp, _i64,
                              ctypes.POINTER(later_opts), ctypes.POINTER(later_info)]),
+    "later_rgsqrf_noalloc": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, ctypes.POINTER(later_opts),
+                                  ctypes.POINTER(later_info)]),
     "later_read_status": (_i, [laterCtxt, _vp, ctypes.POINTER(_i)]),
     "later_caqr_panel": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _vp, _i64, _i]),
     "later_gemm_workspace": (_i, [_i, _i, _i, _i, _i, _i, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
