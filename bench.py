@@ -432,8 +432,8 @@ def main():
     if not args.no_extra:
         extra = {
             "C1": single_config(later, torch, "C1", 8192, 1024, "uniform", ("reference", "mixed")),
-            "C3a": single_config(later, torch, "C3a", 1048576, 128, "normal", ("reference", "mixed")),
-            "C3b": single_config(later, torch, "C3b", 262144, 256, "normal", ("reference", "mixed")),
+            "C3a": single_config(later, torch, "C3a", 1048576, 128, "normal", ("reference", "mixed", "fp8", "bf16")),
+            "C3b": single_config(later, torch, "C3b", 262144, 256, "normal", ("reference", "mixed", "fp8", "bf16")),
             "C4": batched_config(later, torch),
             "C5": serving_config(later, torch),
         }

@@ -63,18 +63,28 @@ enum {
     LATER_GEMM_BF16 = 2,  /* BF16 operands, FP32 accumulate */
     LATER_GEMM_F16X3 = 3, /* FP16 hi/lo split of both operands, three MFMA
                              products, power-of-two column scaling */
-    LATER_GEMM_F16X6 = 4  /* FP16 three-way split (2^-33 representation), six
+    LATER_GEMM_F16X6 = 4, /* FP16 three-way split (2^-33 representation), six
                              MFMA products; projections only (updates use X3) */
+    LATER_GEMM_FP8 = 5    /* FP8 e4m3 operands, each operand column scaled by a
+                             power of two per 128 rows of K (the E8M0 block
+                             scales of v_mfma_scale_f32_32x32x64_f8f6f4), FP32
+                             accumulation; projections only (updates default
+                             to X3); the operands are staged in hwork, which
+                             this precision requires */
 };
 
 /* Panel (leaf) algorithm. */
 enum {
     LATER_PANEL_MGS = 0,    /* reference form: TSQR tree of block MGS, FP32 sums */
     LATER_PANEL_MGS64 = 1,  /* same tree, FP64 column norms and dot products */
-    LATER_PANEL_CHOLQR2 = 2 /* Cholesky QR twice with FP64 Gram factors: three
+    LATER_PANEL_CHOLQR2 = 2,/* Cholesky QR twice with FP64 Gram factors: three
                                grid-wide launches, no sequential column loop;
                                leaves of up to 64 columns (one pass in the
                                second factorisation of LATER_REORTH_PASS) */
+    LATER_PANEL_CHOLQR1 = 3 /* the same leaves with one Cholesky QR pass in
+                               every factorisation (orthogonality like
+                               kappa(leaf)^2 u_64 plus the FP32 rounding of Q:
+                               for a first pass that a second pass repairs) */
 };
 
 /* Re-orthogonalisation (bit flags; NODE|PASS = 3 combines both). */
@@ -92,6 +102,10 @@ typedef struct later_opts {
     int nmin;         /* nodes of width <= nmin use FP32 products (reference: 128) */
     int chunk;        /* rows per FP32 accumulation chunk folded into FP64 (0 = off) */
     int chunk2;       /* the same for the second pass (0 = use chunk) */
+    int gemm_update;  /* LATER_GEMM_* of the node updates A2 -= Q1 R12 above
+                         nmin (F32 .. F16X3), or -1: paired with the
+                         projection precision (F16X6 and FP8 -> F16X3, the
+                         others the same) */
 } later_opts;
 
 /* Defaults reproduce the reference form: FP16 GEMMs above 128 columns, FP32
@@ -99,8 +113,8 @@ typedef struct later_opts {
 void later_opts_default(later_opts *o);
 /* The accuracy-recovering mixed-precision form: FP16X3 first pass (FP64
  * folds every 256 rows), FP16X6 second-pass projections (folds every 32
- * rows), Cholesky QR leaves of up to 64 columns (twice in the first pass,
- * once in the second), second full pass. */
+ * rows), Cholesky QR leaves of up to 64 columns (one pass per leaf,
+ * LATER_PANEL_CHOLQR1), second full pass. */
 void later_opts_mixed(later_opts *o);
 /* The plain-FP32 form: FP32 MFMA everywhere, reference panel, one pass. */
 void later_opts_fp32(later_opts *o);

@@ -21,8 +21,8 @@ LIB_PATH = os.path.join(_HERE, "liblater_amd.so")
 
 # Status codes (include/later.h)
 OK, ERR_SHAPE, ERR_WORKSPACE, ERR_OPTION, ERR_DEVICE, ERR_BREAKDOWN, ERR_ALIAS, ERR_BUDGET = range(8)
-GEMM_F32, GEMM_F16, GEMM_BF16, GEMM_F16X3, GEMM_F16X6 = range(5)
-PANEL_MGS, PANEL_MGS64, PANEL_CHOLQR2 = range(3)
+GEMM_F32, GEMM_F16, GEMM_BF16, GEMM_F16X3, GEMM_F16X6, GEMM_FP8 = range(6)
+PANEL_MGS, PANEL_MGS64, PANEL_CHOLQR2, PANEL_CHOLQR1 = range(4)
 REORTH_NONE, REORTH_NODE, REORTH_PASS = range(3)
 
 _STATUS_NAMES = {
@@ -45,11 +45,12 @@ class laterCtxt(ctypes.Structure):
 class later_opts(ctypes.Structure):
     _fields_ = [("gemm", ctypes.c_int), ("gemm_pass2", ctypes.c_int), ("panel", ctypes.c_int),
                 ("reorth", ctypes.c_int), ("nmin", ctypes.c_int), ("chunk", ctypes.c_int),
-                ("chunk2", ctypes.c_int)]
+                ("chunk2", ctypes.c_int), ("gemm_update", ctypes.c_int)]
 
     def __repr__(self):
-        return ("later_opts(gemm=%d, gemm_pass2=%d, panel=%d, reorth=%d, nmin=%d, chunk=%d, chunk2=%d)"
-                % (self.gemm, self.gemm_pass2, self.panel, self.reorth, self.nmin, self.chunk, self.chunk2))
+        return ("later_opts(gemm=%d, gemm_pass2=%d, panel=%d, reorth=%d, nmin=%d, chunk=%d, chunk2=%d, "
+                "gemm_update=%d)" % (self.gemm, self.gemm_pass2, self.panel, self.reorth, self.nmin, self.chunk,
+                                     self.chunk2, self.gemm_update))
 
 
 class later_info(ctypes.Structure):
@@ -137,11 +138,20 @@ def lib() -> ctypes.CDLL:
 # --------------------------------------------------------------------------
 def opts(form: str = "reference", **over) -> later_opts:
     """'reference' (FP16 node GEMMs, FP32 MGS panel, one pass), 'fp32'
-    (FP32 MFMA everywhere) or 'mixed' (FP16 hi/lo products, FP64 panel sums,
-    second pass).  Keyword arguments override single fields."""
+    (FP32 MFMA everywhere), 'mixed' (FP16 hi/lo products, FP64 panel sums,
+    second pass), or 'fp8' / 'bf16': the mixed form whose first-pass
+    projections run e4m3 (per-128-row block scales) or BF16 operands, with
+    hi/lo updates and nodes above 32 columns (for well-conditioned tall
+    panels; the second pass is the refinement).  Keyword arguments override
+    single fields."""
     o = later_opts()
-    fn = {"reference": "later_opts_default", "fp32": "later_opts_fp32", "mixed": "later_opts_mixed"}[form]
+    base = {"fp8": "mixed", "bf16": "mixed"}.get(form, form)
+    fn = {"reference": "later_opts_default", "fp32": "later_opts_fp32", "mixed": "later_opts_mixed"}[base]
     getattr(lib(), fn)(ctypes.byref(o))
+    if form in ("fp8", "bf16"):
+        o.gemm = GEMM_FP8 if form == "fp8" else GEMM_BF16
+        o.gemm_update = GEMM_F16X3
+        o.nmin = 32
     for k, v in over.items():
         setattr(o, k, int(v))
     return o

@@ -362,8 +362,9 @@ LATER_API int later_batch_plan_create(int count, const int *m, const int *n, flo
     // The same option checks as later_rgsqrf_ex (a chunk below 32 would fold
     // the FP64 sums only after the last k).
     auto prec_ok = [](int p) { return p >= LATER_GEMM_F32 && p <= LATER_GEMM_F16X6; };
-    if (!prec_ok(o.gemm) || !prec_ok(o.gemm_pass2) || o.panel < 0 || o.panel > 2 || o.reorth < 0 || o.reorth > 3 ||
-        o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0 || o.chunk2 < 0 || (o.chunk2 % 32) != 0)
+    if (!prec_ok(o.gemm) || !prec_ok(o.gemm_pass2) || o.gemm_update != -1 || o.panel < 0 || o.panel > 3 ||
+        o.reorth < 0 || o.reorth > 3 || o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0 || o.chunk2 < 0 ||
+        (o.chunk2 % 32) != 0)
         return LATER_ERR_OPTION;
     int64_t need = 0;
     int rc = later_batch_workspace(count, m, n, &o, &need);

@@ -3,7 +3,7 @@
 #include "common.h"
 
 enum { GEMM_F32 = LATER_GEMM_F32, GEMM_F16 = LATER_GEMM_F16, GEMM_BF16 = LATER_GEMM_BF16,
-       GEMM_F16X3 = LATER_GEMM_F16X3, GEMM_F16X6 = LATER_GEMM_F16X6 };
+       GEMM_F16X3 = LATER_GEMM_F16X3, GEMM_F16X6 = LATER_GEMM_F16X6, GEMM_FP8 = LATER_GEMM_FP8 };
 
 // One product of the tiled MFMA kernel (device-side argument block; the
 // grouped launches read an array of these).
@@ -25,13 +25,31 @@ struct GemmArgs {
     // Ah + c * psa), already scaled by 2^a_exp / 2^b_exp[j].
     const uint16_t *Ah; int64_t ldah, psa;
     const uint16_t *Bh; int64_t ldbh, psb;
+    // FP8 operands (GEMM_FP8): Ah / Bh hold bytes (ld in bytes), the E8M0
+    // block scales are at byte offset psa / psb with row stride lsa / lsb
+    // (scale of row i, K block kb at kb * ls + i).
+    int64_t lsa, lsb;
 };
 
-// One operand in pre-split form (see gemm_split).
+// One operand in pre-split form (see gemm_split, gemm_split8).
 struct Pieces {
     const uint16_t *p;
     int64_t ld, stride;
+    int64_t ls;  // FP8 only: row stride of the block scales
 };
+
+// FP8 staging (GEMM_FP8) of an operand whose K runs down its columns: column
+// j of X (K rows) becomes K (padded with zeros to a multiple of 64) e4m3
+// bytes at buf + j * fp8_ld(K), scaled by 2^-e for each 128-row block kb, and
+// the E8M0 byte 127 - e of that block sits at buf + fp8_scale_off(K, cols) +
+// kb * fp8_ls(cols) + j.  e makes the block's largest magnitude land in
+// (224, 448], e4m3's top binade.
+int64_t fp8_ld(int K);
+int64_t fp8_ls(int cols);
+int64_t fp8_scale_off(int K, int cols);
+int64_t fp8_bytes(int K, int cols);  // whole staging buffer, a multiple of 256
+Pieces fp8_pieces(const void *buf, int K, int cols);
+int gemm_split8(hipStream_t s, int K, int cols, const float *X, int64_t ldx, void *buf, int *launches);
 
 struct ColexpArgs {
     int m;
@@ -50,7 +68,7 @@ int colexp_grouped(hipStream_t s, const ColexpArgs *args, const int2 *tasks, int
 // Floats of workspace the projection needs for its split-K partials.
 int64_t gemm_proj_work_floats(int m, int n1, int n2, int chunk);
 
-// 16-bit pieces per value of a precision (1, 2 or 3; 0 for FP32).
+// 16-bit pieces per value of a precision (1, 2 or 3; 0 for FP32 and FP8).
 int gemm_pieces(int prec);
 
 // Splits X (rows x cols FP32, column-major, ldx), scaled by 2^a_exp (col_exp

@@ -808,6 +808,172 @@ __global__ __launch_bounds__(128 * WGM) void gemm_lds_kernel(GemmArgs g)
     gemm_lds_body<PREC, CH64, BM, WGM>(g, tm, tn, split);
 }
 
+// FP8 (e4m3) products on staged operands: the streaming kernel's LDS ring
+// with 64 bytes per row and slab (64 k instead of 32), one
+// v_mfma_scale_f32_32x32x64_f8f6f4 per 32 x 32 block and slab, and the E8M0
+// scales of the slab's 128-row K block streamed beside the data (one dword
+// load per lane by waves 0 and 1: the tile's row scales are contiguous).
+// Lane l of a fragment supplies row l % 32 and k = 32 (l / 32) .. + 31 (two
+// 16-byte chunks), with the scale of that row; both operands use the same k
+// order, so the contraction is over matching k.  BM x 128 tiles, eight waves
+// of (BM / 4) x 64, FP32 accumulation, split-K partials in FP32.
+template <int BM>
+__device__ __forceinline__ void gemm8_body(const GemmArgs &g, const int tm, const int tn, const int split)
+{
+    constexpr int NW = 8, WGM = 4, BN = 128;
+    constexpr int BK = 64, WM = BM / WGM, WN = 64, FM = WM / 32, FN = 2;
+    constexpr int BPA = BM / 16 / NW, BPB = BN / 16 / NW;  // 1 KB blocks (16 rows x 64 B) per wave
+    static_assert(BPA >= 1 && BPB >= 1 && FM >= 1, "tile shape");
+    constexpr int PBA = BM * 64, PBB = BN * 64, SCB = 256;  // bytes per stage: A, B, each scale row
+    constexpr int SB = PBA + PBB + 2 * SCB;
+    constexpr int NS = 4;
+    static_assert(NS * SB <= 163840, "LDS");
+    constexpr int LPS = BPA + BPB;  // data loads per wave per stage (+1 scale load on waves 0 and 1)
+    typedef int v8i __attribute__((ext_vector_type(8)));
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    __shared__ __attribute__((aligned(16))) char lds[NS * SB];
+
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int kbeg = split * g.kper;
+    const int kend = min(g.K, kbeg + g.kper);
+    const int ntiles = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = wave >> 1, wn = wave & 1;
+    const uint8_t *Pa = (const uint8_t *)g.Ah, *Pb = (const uint8_t *)g.Bh;
+
+    auto issue = [&](int slab, int stage) {
+        const int k0 = kbeg + slab * BK;
+        char *sbase = lds + stage * SB;
+#pragma unroll
+        for (int op = 0; op < 2; ++op) {
+            const uint8_t *P = op ? Pb : Pa;
+            const int64_t ld = op ? g.ldbh : g.ldah;
+            const int r0 = op ? n0 : m0, rmax = op ? g.N : g.M;
+            char *obase = sbase + (op ? PBA : 0);
+#pragma unroll
+            for (int i = 0; i < (op ? BPB : BPA); ++i) {
+                const int blk = (op ? BPB : BPA) * wave + i;
+                const int q = blk * 64 + lane;
+                const int r = q >> 2;
+                const int cg = (q & 3) ^ ((r >> 2) & 3);
+                const uint8_t *src = P + (int64_t)min(r0 + r, rmax - 1) * ld + k0 + 16 * cg;
+                __builtin_amdgcn_global_load_lds((const void *)src,
+                                                 (__attribute__((address_space(3))) void *)(obase + blk * 1024), 16,
+                                                 0, 0);
+            }
+        }
+        if (wave < 2) {
+            // Scales of rows r0 .. r0 + 255 of K block k0 / 128 (the scale
+            // rows are padded to a multiple of 256, so every dword is in
+            // bounds; rows past M or N only feed outputs that are not stored).
+            const int kb = k0 >> 7;
+            const uint8_t *S = (wave ? Pb + g.psb + (int64_t)kb * g.lsb + n0 : Pa + g.psa + (int64_t)kb * g.lsa + m0);
+            __builtin_amdgcn_global_load_lds((const void *)(S + 4 * lane),
+                                             (__attribute__((address_space(3))) void *)(sbase + PBA + PBB +
+                                                                                        wave * SCB),
+                                             4, 0, 0);
+        }
+    };
+    auto frag = [&](const char *pbase, int row) -> v8i {
+        const int sw = (row >> 2) & 3, c0 = 2 * (lane >> 5);
+        const v4i x = *(const v4i *)(pbase + row * 64 + ((c0 ^ sw) * 16));
+        const v4i y = *(const v4i *)(pbase + row * 64 + (((c0 + 1) ^ sw) * 16));
+        return v8i{x[0], x[1], x[2], x[3], y[0], y[1], y[2], y[3]};
+    };
+
+    floatx16 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+#pragma unroll
+    for (int st = 0; st < NS - 1; ++st)
+        if (st < ntiles) issue(st, st);
+    for (int it = 0; it < ntiles; ++it) {
+        if (it + NS - 2 < ntiles) {
+            if (wave < 2) wait_vmcnt<(NS - 2) * (LPS + 1)>();
+            else wait_vmcnt<(NS - 2) * LPS>();
+        } else {
+            wait_vmcnt<0>();
+        }
+        asm volatile("s_barrier" ::: "memory");
+        if (it + NS - 1 < ntiles) issue(it + NS - 1, (it + NS - 1) % NS);
+        const char *sb = lds + (it % NS) * SB;
+        const uint8_t *sca = (const uint8_t *)(sb + PBA + PBB), *scb = sca + SCB;
+        v8i fa[FM], fb[FN];
+        int sa[FM], sbv[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            const int row = wm * WM + 32 * i + (lane & 31);
+            fa[i] = frag(sb, row);
+            sa[i] = sca[row];
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int row = wn * WN + 32 * j + (lane & 31);
+            fb[j] = frag(sb + PBA, row);
+            sbv[j] = scb[row];
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fb[j], fa[i], acc[i][j], 0, 0, 0, sbv[j],
+                                                                            0, sa[i]);
+    }
+
+    store_tile<false, FM, FN>(g, split, m0 + wm * WM, n0 + wn * WN,
+                              [&](int i, int j, int r) -> double { return (double)acc[i][j][r]; });
+}
+
+template <int BM>
+__global__ __launch_bounds__(512) void gemm8_kernel(GemmArgs g)
+{
+    int tm, tn, split;
+    tile_of(tm, tn, split);
+    gemm8_body<BM>(g, tm, tn, split);
+}
+
+// FP32 -> e4m3 staging (see fp8_ld): 32 lanes per 128-row block of one
+// column, four rows per lane; the block's largest magnitude (over the 32
+// lanes) sets the power of two e, and the bytes are v_cvt_pk_fp8_f32's
+// round-to-nearest-even of x 2^e.  Rows K .. roundup(K, 64) - 1 are zeros.
+__global__ __launch_bounds__(256) void split8_kernel(int K, const float *X, int64_t ldx, uint8_t *out, int64_t ldo,
+                                                     uint8_t *sc, int64_t ls)
+{
+    const int j = blockIdx.y;
+    const int k = (blockIdx.x * 256 + threadIdx.x) * 4;
+    const int prow = (K + 63) / 64 * 64;
+    if (blockIdx.x * 1024 >= prow) return;  // uniform over the workgroup
+    const float *x = X + (int64_t)j * ldx;
+    float v[4];
+    if (k + 4 <= K && ((((uintptr_t)(x + k)) & 15) == 0)) {
+        const floatx4 a = *(const floatx4 *)(x + k);
+        v[0] = a[0]; v[1] = a[1]; v[2] = a[2]; v[3] = a[3];
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = (k + e < K) ? x[k + e] : 0.0f;
+    }
+    float mx = fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3])));
+#pragma unroll
+    for (int mask = 1; mask < 32; mask <<= 1) mx = fmaxf(mx, __shfl_xor(mx, mask, 32));
+    int e = 0;
+    if (mx > 0.0f && mx < INFINITY) {
+        int fe;
+        const float f = frexpf(mx, &fe);  // mx = f 2^fe, f in [0.5, 1); 448 = 0.875 2^9
+        e = (f <= 0.875f) ? 9 - fe : 8 - fe;
+        e = e < -126 ? -126 : (e > 126 ? 126 : e);
+    }
+    const float s = ldexpf(1.0f, e);
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * s, v[1] * s, 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * s, v[3] * s, w, true);
+    if (k < prow) *(int *)(out + (int64_t)j * ldo + k) = w;
+    if ((threadIdx.x & 31) == 0 && k < K) sc[(int64_t)(k >> 7) * ls + j] = (uint8_t)(127 - e);
+}
+
 // Grouped launch: workgroup b computes tile (t.y, t.z) of split t.w of the
 // product described by args[t.x], t = tasks[b].
 template <int PREC, int BM, int BN, bool AKM, bool CH64>
@@ -1107,6 +1273,20 @@ int dispatch(hipStream_t s, int prec, const GemmArgs &g, int nsplit, bool small)
     LATER_DISPATCH_16(GEMM_F16X3)
     LATER_DISPATCH_16(GEMM_F16X6)
 #undef LATER_DISPATCH_16
+    case GEMM_FP8: {
+        // Staged e4m3 operands only; 256-row tiles when they still give every
+        // CU a tile.
+        if (!pre || !AKM) return LATER_ERR_WORKSPACE;
+        const int64_t t256 = ceil_div64(g.M, 256) * ceil_div64(g.N, 128) * nsplit;
+        if (t256 >= 256) {
+            const dim3 grid((unsigned)ceil_div64(g.M, 256), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
+            hipLaunchKernelGGL(gemm8_kernel<256>, grid, dim3(512), 0, s, g);
+        } else {
+            const dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
+            hipLaunchKernelGGL(gemm8_kernel<128>, grid, dim3(512), 0, s, g);
+        }
+        break;
+    }
     default: return LATER_ERR_OPTION;
     }
     HIP_OK(hipGetLastError());
@@ -1174,11 +1354,12 @@ static void proj_plan(int m, int n1, int n2, int chunk, bool *small, int *nsplit
     const int64_t maxs = m / 256 > 1 ? m / 256 : 1;
     if (ns > maxs) ns = maxs;
     const int64_t per = (int64_t)n1 * n2 * (chunk > 0 ? 2 : 1);
-    // The 64 Ki floats held back keep the partials, with the rest of the
-    // factorisation's FP32 workspace, inside m n / 2 floats: the reference's
-    // callers size the 16-bit arena at m n halves (test/test_qr.cu:76,
-    // EVD/later_qdwh_polar.cu:75), which then holds everything.
-    const int64_t budget = ((int64_t)m * (n1 + n2) / (thin ? 8 : 2) - 65536) / per;
+    // The 64 Ki floats held back from the wide budget keep the partials, with
+    // the rest of the factorisation's FP32 workspace, inside m n / 2 floats:
+    // the reference's callers size the 16-bit arena at m n halves
+    // (test/test_qr.cu:76, EVD/later_qdwh_polar.cu:75), which then holds
+    // everything.  Thin products stay within m n / 8 without it.
+    const int64_t budget = thin ? (int64_t)m * (n1 + n2) / 8 / per : ((int64_t)m * (n1 + n2) / 2 - 65536) / per;
     if (ns > budget) ns = budget;
     if (ns < 1) ns = 1;
     int64_t kp = ceil_div64(m, ns);
@@ -1202,15 +1383,18 @@ static void set_pieces(GemmArgs &g, const Pieces *ah, const Pieces *bh)
     g.Ah = pre ? ah->p : nullptr;
     g.ldah = pre ? ah->ld : 0;
     g.psa = pre ? ah->stride : 0;
+    g.lsa = pre ? ah->ls : 0;
     g.Bh = pre ? bh->p : nullptr;
     g.ldbh = pre ? bh->ld : 0;
     g.psb = pre ? bh->stride : 0;
+    g.lsb = pre ? bh->ls : 0;
 }
 
 int gemm_proj(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, int64_t ldq, const float *B,
               int64_t ldb, float *C, int64_t ldc, int chunk, const int *b_exp, int a_exp, float *work,
               int64_t wfloats, int *launches, const Pieces *qh, const Pieces *bh)
 {
+    if (prec == GEMM_FP8) chunk = 0;  // exact e4m3 products, FP32 sums: no FP64 folds
     const bool ch64 = chunk > 0;
     if (ch64 && (chunk % 32 || (chunk < kMaxBK && kMaxBK % chunk))) return LATER_ERR_OPTION;
     bool small;
@@ -1329,6 +1513,38 @@ int gemm_split(hipStream_t s, int prec, int rows, int cols, const float *X, int6
     case GEMM_F16X6: launch_split<GEMM_F16X6>(s, rows, cols, X, ldx, a_exp, col_exp, out, ldo, stride, trans); break;
     default: return LATER_ERR_OPTION;
     }
+    HIP_OK(hipGetLastError());
+    if (launches) ++*launches;
+    return LATER_OK;
+}
+
+int64_t fp8_ld(int K) { return ((int64_t)K + 63) / 64 * 64 + 64; }
+int64_t fp8_ls(int cols) { return ((int64_t)cols + 255) / 256 * 256 + 256; }
+int64_t fp8_scale_off(int K, int cols) { return (fp8_ld(K) * cols + 255) / 256 * 256; }
+int64_t fp8_bytes(int K, int cols)
+{
+    return fp8_scale_off(K, cols) + ((int64_t)K + 127) / 128 * fp8_ls(cols);
+}
+
+Pieces fp8_pieces(const void *buf, int K, int cols)
+{
+    Pieces p;
+    p.p = (const uint16_t *)buf;
+    p.ld = fp8_ld(K);
+    p.stride = fp8_scale_off(K, cols);
+    p.ls = fp8_ls(cols);
+    return p;
+}
+
+int gemm_split8(hipStream_t s, int K, int cols, const float *X, int64_t ldx, void *buf, int *launches)
+{
+    if (K <= 0 || cols <= 0) return LATER_OK;
+    if (cols > 65535) return LATER_ERR_SHAPE;
+    uint8_t *b = (uint8_t *)buf;
+    const int prow = (K + 63) / 64 * 64;
+    const dim3 grid((unsigned)ceil_div64(prow, 1024), (unsigned)cols);
+    hipLaunchKernelGGL(split8_kernel, grid, dim3(256), 0, s, K, X, ldx, b, fp8_ld(K), b + fp8_scale_off(K, cols),
+                       fp8_ls(cols));
     HIP_OK(hipGetLastError());
     if (launches) ++*launches;
     return LATER_OK;

@@ -28,10 +28,14 @@ constexpr int kQExp = 14;          // |Q| <= 1: Q scaled by 2^14 before 16-bit c
 
 int split_n(int n) { return 32 * ((n + 63) / 64); }
 
-bool prec_ok(int p) { return p >= LATER_GEMM_F32 && p <= LATER_GEMM_F16X6; }
+bool prec_ok(int p) { return p >= LATER_GEMM_F32 && p <= LATER_GEMM_FP8; }
+bool chol_panel(int p) { return p == LATER_PANEL_CHOLQR2 || p == LATER_PANEL_CHOLQR1; }
 bool scaled(int p) { return p == LATER_GEMM_F16X3 || p == LATER_GEMM_F16X6; }
-// F16X6 is a projection precision; its updates use the hi/lo pair.
-int update_prec(int p) { return p == LATER_GEMM_F16X6 ? LATER_GEMM_F16X3 : p; }
+// F16X6 and FP8 are projection precisions; their updates use the hi/lo pair.
+int update_prec(int p) { return (p == LATER_GEMM_F16X6 || p == LATER_GEMM_FP8) ? LATER_GEMM_F16X3 : p; }
+// The update precision of the nodes whose projections run `p`.
+int update_of(const later_opts &o, int p) { return o.gemm_update >= 0 ? o.gemm_update : update_prec(p); }
+bool update_ok(int p) { return p == -1 || (p >= LATER_GEMM_F32 && p <= LATER_GEMM_F16X3); }
 
 struct Layout {
     int64_t header, exps, panel, gemm, snode, r2, tcopy, total;
@@ -50,15 +54,21 @@ int64_t piece_ld(int64_t x) { return (x + 7) / 8 * 8 + 64; }
 // Pieces of the projection operands.
 int proj_pieces(int prec) { return gemm_pieces(prec); }
 
-// 16-bit elements the pre-split operands of one node need: Q1 (column-major
-// and transposed), A2 and R12.
-int64_t node_pieces(int m, int n, int prec)
+// 16-bit elements the pre-split operands of one node need: the e4m3 bytes
+// of Q1 and A2 for FP8 projections (any width above nmin); for nodes of
+// kPieceMinN columns and more, Q1 and A2 as 16-bit pieces for the projection
+// and Q1 (transposed) and R12 for the update.
+int64_t node_pieces(int m, int n, int p, int pu)
 {
-    const int np = proj_pieces(prec), npu = gemm_pieces(update_prec(prec));
-    if (npu == 0 || n < kPieceMinN) return 0;
     const int n1 = split_n(n), n2 = n - n1;
     const int64_t ldm = piece_ld(m), ldn1 = piece_ld(n1);
-    return np * ldm * (n1 + n2) + npu * ldn1 * ((int64_t)m + n2);
+    int64_t tot = 0;
+    if (p == LATER_GEMM_FP8) tot += (fp8_bytes(m, n1) + fp8_bytes(m, n2)) / 2;
+    if (n >= kPieceMinN && gemm_pieces(pu) > 0) {
+        if (p != LATER_GEMM_FP8) tot += proj_pieces(p) * ldm * (n1 + n2);
+        tot += gemm_pieces(pu) * ldn1 * ((int64_t)m + n2);
+    }
+    return tot;
 }
 
 // Largest split-K partial buffer over the nodes of the recursion.
@@ -78,7 +88,7 @@ Layout layout(int m, int n, const later_opts &o)
     Layout L;
     L.header = kHeaderFloats;
     L.exps = ((int64_t)n + 63) / 64 * 64;
-    L.panel = (o.panel == LATER_PANEL_CHOLQR2) ? panel_chol_work_floats(n) : panel_work_floats_any(m);
+    L.panel = chol_panel(o.panel) ? panel_chol_work_floats(n) : panel_work_floats_any(m);
     int64_t g = 0;
     walk_gemm(m, n, o.nmin, o.chunk, &g);
     if (o.reorth & LATER_REORTH_PASS) walk_gemm(m, n, o.nmin, o.chunk2 > 0 ? o.chunk2 : o.chunk, &g);
@@ -89,14 +99,14 @@ Layout layout(int m, int n, const later_opts &o)
     L.tcopy = L.r2;
     L.total = L.header + L.exps + L.panel + L.gemm + L.snode + L.r2 + L.tcopy;
     // The top node is the largest.
-    int64_t ph = (o.gemm != LATER_GEMM_F32 && n > o.nmin) ? node_pieces(m, n, o.gemm) : 0;
+    int64_t ph = (o.gemm != LATER_GEMM_F32 && n > o.nmin) ? node_pieces(m, n, o.gemm, update_of(o, o.gemm)) : 0;
     if ((o.reorth & LATER_REORTH_PASS) && o.gemm_pass2 != LATER_GEMM_F32 && n > o.nmin) {
-        const int64_t p2 = node_pieces(m, n, o.gemm_pass2);
+        const int64_t p2 = node_pieces(m, n, o.gemm_pass2, update_of(o, o.gemm_pass2));
         if (p2 > ph) ph = p2;
     }
     // The final R = R1 + (R2 - I) R1 stages R2 (transposed) and R1.
     if ((o.reorth & LATER_REORTH_PASS) && o.gemm_pass2 != LATER_GEMM_F32 && n >= kPieceMinN) {
-        const int64_t pt = 2 * gemm_pieces(update_prec(o.gemm_pass2)) * piece_ld(n) * n;
+        const int64_t pt = 2 * gemm_pieces(update_of(o, o.gemm_pass2)) * piece_ld(n) * n;
         if (pt > ph) ph = pt;
     }
     L.hpieces = ph;
@@ -166,74 +176,92 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     if (depth > r.depth) r.depth = depth;
     // Cholesky QR leaves take up to 64 columns (one Gram and factor for what
     // would otherwise be two leaves and a node); the MGS panels take 32.
-    if (n <= (r.o.panel == LATER_PANEL_CHOLQR2 ? 64 : 32)) {
-        if (r.o.panel == LATER_PANEL_CHOLQR2)
+    if (n <= (chol_panel(r.o.panel) ? 64 : 32)) {
+        if (chol_panel(r.o.panel))
             return panel_cholqr2(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n, r.status, &r.launches,
-                                 r.chol_once);
+                                 r.chol_once || r.o.panel == LATER_PANEL_CHOLQR1);
         return panel_tsqr(r.s, m, n, A, (int)lda, R, (int)ldr, r.panel, r.panel_n,
                           r.o.panel == LATER_PANEL_MGS64, r.status, &r.launches);
     }
     const int n1 = split_n(n), n2 = n - n1;
     const bool inner = (n <= r.o.nmin);
-    const int p = inner ? LATER_GEMM_F32 : prec;
+    const int p = inner ? LATER_GEMM_F32 : prec;               // projection precision
+    const int pu = inner ? LATER_GEMM_F32 : update_of(r.o, prec);  // update precision
     const int chunk = inner ? 0 : r.chunk;
     const bool f64 = chunk > 0;
     int rc = qr_rec(r, m, n1, A, lda, R, ldr, prec, depth + 1);
     if (rc) return rc;
     float *A2 = A + (int64_t)n1 * lda;
     float *R12 = R + (int64_t)n1 * ldr;
-    const int aexp = scaled(p) ? kQExp : 0;
-    const int *bexp = nullptr;
+    const int aexp = scaled(p) ? kQExp : 0, aexp_u = scaled(pu) ? kQExp : 0;
+    const int *bexp = scaled(p) ? r.exps : nullptr, *bexp_u = scaled(pu) ? r.exps : nullptr;
     // Pre-split operands (the reference's s2h staging, util/util.cu:24-32,
     // as FP16 pieces): the same MFMA inputs as the fused conversion, made
-    // once per node instead of once per tile.
-    const bool pre = r.pcs && !inner && n >= kPieceMinN && gemm_pieces(update_prec(p)) > 0;
-    const bool pre_proj = pre && proj_pieces(p) > 0;
+    // once per node instead of once per tile.  FP8 projections are always
+    // staged (their block scales need a pass over each 128-row block).
+    const bool fp8 = p == LATER_GEMM_FP8;
+    const bool pre = r.pcs && !inner && n >= kPieceMinN && gemm_pieces(pu) > 0;
+    const bool pre_proj = fp8 ? (r.pcs && !inner) : (pre && proj_pieces(p) > 0);
+    if (fp8 && !pre_proj) return LATER_ERR_WORKSPACE;
     Pieces q1h{}, a2h{}, q1t{}, r12h{};
-    if (pre) {
-        const int64_t ldm = piece_ld(m), ldn1 = piece_ld(n1);
+    {
         uint16_t *b = r.pcs;
-        q1h = {b, ldm, ldm * n1}; b += proj_pieces(p) * ldm * n1;
-        a2h = {b, ldm, ldm * n2}; b += proj_pieces(p) * ldm * n2;
-        q1t = {b, ldn1, ldn1 * m}; b += gemm_pieces(update_prec(p)) * ldn1 * m;
-        r12h = {b, ldn1, ldn1 * n2};
+        if (fp8) {
+            q1h = fp8_pieces(b, m, n1);
+            b += fp8_bytes(m, n1) / 2;
+            a2h = fp8_pieces(b, m, n2);
+            b += fp8_bytes(m, n2) / 2;
+        } else if (pre_proj) {
+            const int64_t ldm = piece_ld(m);
+            q1h = {b, ldm, ldm * n1}; b += proj_pieces(p) * ldm * n1;
+            a2h = {b, ldm, ldm * n2}; b += proj_pieces(p) * ldm * n2;
+        }
+        if (pre) {
+            const int64_t ldn1 = piece_ld(n1);
+            q1t = {b, ldn1, ldn1 * m}; b += gemm_pieces(pu) * ldn1 * m;
+            r12h = {b, ldn1, ldn1 * n2};
+        }
     }
-    if (pre_proj && (rc = gemm_split(r.s, p, m, n1, A, lda, aexp, nullptr, (uint16_t *)q1h.p, q1h.ld, q1h.stride,
-                                     false, &r.launches)))
-        return rc;
     // R12 = Q1^T A2   (QR/later_rgsqrf.cu:43-49); A2's column exponents and
     // pieces in one launch when it is staged.
-    if (scaled(p)) bexp = r.exps;
-    if (pre_proj && scaled(p)) {
-        if ((rc = gemm_split_exp(r.s, p, m, n2, A2, lda, r.exps, (uint16_t *)a2h.p, a2h.ld, a2h.stride,
-                                 &r.launches)))
-            return rc;
+    if (fp8) {
+        if ((rc = gemm_split8(r.s, m, n1, A, lda, (void *)q1h.p, &r.launches))) return rc;
+        if ((rc = gemm_split8(r.s, m, n2, A2, lda, (void *)a2h.p, &r.launches))) return rc;
     } else {
-        if (scaled(p) && (rc = gemm_colexp(r.s, m, n2, A2, lda, r.exps, &r.launches))) return rc;
-        if (pre_proj && (rc = gemm_split(r.s, p, m, n2, A2, lda, aexp, bexp, (uint16_t *)a2h.p, a2h.ld, a2h.stride,
-                                         false, &r.launches)))
+        if (pre_proj && (rc = gemm_split(r.s, p, m, n1, A, lda, aexp, nullptr, (uint16_t *)q1h.p, q1h.ld,
+                                         q1h.stride, false, &r.launches)))
             return rc;
+        if (pre_proj && scaled(p)) {
+            if ((rc = gemm_split_exp(r.s, p, m, n2, A2, lda, r.exps, (uint16_t *)a2h.p, a2h.ld, a2h.stride,
+                                     &r.launches)))
+                return rc;
+        } else {
+            if (scaled(p) && (rc = gemm_colexp(r.s, m, n2, A2, lda, r.exps, &r.launches))) return rc;
+            if (pre_proj && (rc = gemm_split(r.s, p, m, n2, A2, lda, aexp, bexp, (uint16_t *)a2h.p, a2h.ld,
+                                             a2h.stride, false, &r.launches)))
+                return rc;
+        }
     }
     rc = gemm_proj(r.s, p, m, n1, n2, A, lda, A2, lda, R12, ldr, chunk, bexp, aexp, r.gemm, r.gemm_n,
                    &r.launches, pre_proj ? &q1h : nullptr, pre_proj ? &a2h : nullptr);
     if (rc) return rc;
-    // A2 -= Q1 R12    (QR/later_rgsqrf.cu:51-56)
+    // A2 -= Q1 R12    (QR/later_rgsqrf.cu:51-56); R12's column exponents
+    // replace A2's.
     if (pre) {
-        const int pu = update_prec(p);
-        if ((rc = gemm_split(r.s, pu, m, n1, A, lda, aexp, nullptr, (uint16_t *)q1t.p, q1t.ld, q1t.stride, true,
+        if ((rc = gemm_split(r.s, pu, m, n1, A, lda, aexp_u, nullptr, (uint16_t *)q1t.p, q1t.ld, q1t.stride, true,
                              &r.launches)))
             return rc;
-        if (scaled(p))
+        if (scaled(pu))
             rc = gemm_split_exp(r.s, pu, n1, n2, R12, ldr, r.exps, (uint16_t *)r12h.p, r12h.ld, r12h.stride,
                                 &r.launches);
         else
-            rc = gemm_split(r.s, pu, n1, n2, R12, ldr, 0, bexp, (uint16_t *)r12h.p, r12h.ld, r12h.stride, false,
+            rc = gemm_split(r.s, pu, n1, n2, R12, ldr, 0, nullptr, (uint16_t *)r12h.p, r12h.ld, r12h.stride, false,
                             &r.launches);
         if (rc) return rc;
-    } else if (scaled(p)) {
+    } else if (scaled(pu)) {
         if ((rc = gemm_colexp(r.s, n1, n2, R12, ldr, r.exps, &r.launches))) return rc;
     }
-    rc = gemm_update(r.s, update_prec(p), m, n1, n2, A, lda, R12, ldr, A2, lda, f64, bexp, aexp, &r.launches,
+    rc = gemm_update(r.s, pu, m, n1, n2, A, lda, R12, ldr, A2, lda, f64, bexp_u, aexp_u, &r.launches,
                      pre ? &q1t : nullptr, pre ? &r12h : nullptr);
     if (rc) return rc;
     if (r.node_reorth) {
@@ -245,10 +273,10 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
         rc = gemm_proj(r.s, p, m, n1, n2, A, lda, A2, lda, S, n1, chunk, bexp, aexp, r.gemm, r.gemm_n,
                        &r.launches);
         if (rc) return rc;
-        if (scaled(p)) {
+        if (scaled(pu)) {
             if ((rc = gemm_colexp(r.s, n1, n2, S, n1, r.exps, &r.launches))) return rc;
         }
-        rc = gemm_update(r.s, update_prec(p), m, n1, n2, A, lda, S, n1, A2, lda, f64, bexp, aexp, &r.launches);
+        rc = gemm_update(r.s, pu, m, n1, n2, A, lda, S, n1, A2, lda, f64, bexp_u, aexp_u, &r.launches);
         if (rc) return rc;
         hipLaunchKernelGGL(add_kernel, grid1((int64_t)n1 * n2), dim3(256), 0, r.s, n1, n2, S, (int64_t)n1, R12,
                            ldr);
@@ -276,6 +304,7 @@ LATER_API void later_opts_default(later_opts *o)
     o->nmin = 128;
     o->chunk = 0;
     o->chunk2 = 0;
+    o->gemm_update = -1;
 }
 
 LATER_API void later_opts_fp32(later_opts *o)
@@ -288,15 +317,17 @@ LATER_API void later_opts_fp32(later_opts *o)
 // Mixed form: FP16 hi/lo products with FP64 folds every 256 rows in the
 // first pass; three-piece products folded every 32 rows in the second pass,
 // whose projections set the orthogonality floor; Cholesky QR leaves of up to
-// 64 columns (twice in the first pass, once in the second).  At 16384^2
-// (N(0,1)) this gives ||I - Q^T Q||_F = 8.6e-6 and ||A - QR||_F / ||A||_F =
-// 3.4e-7; the same with two-piece second-pass products gives 1.3e-5.
+// 64 columns, one pass each (the second factorisation repairs what a second
+// Cholesky QR pass of a first-pass leaf would: at 16384^2 the orthogonality
+// is 8.77e-6 either way).  At 16384^2 (N(0,1)) this gives
+// ||I - Q^T Q||_F = 8.8e-6 and ||A - QR||_F / ||A||_F = 3.4e-7; the same
+// with two-piece second-pass products gives 1.3e-5.
 LATER_API void later_opts_mixed(later_opts *o)
 {
     later_opts_default(o);
     o->gemm = LATER_GEMM_F16X3;
     o->gemm_pass2 = LATER_GEMM_F16X6;
-    o->panel = LATER_PANEL_CHOLQR2;
+    o->panel = LATER_PANEL_CHOLQR1;
     o->reorth = LATER_REORTH_PASS;
     o->chunk = 256;
     o->chunk2 = 32;
@@ -322,9 +353,10 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     if (info) memset(info, 0, sizeof(*info));
     int rc = LATER_OK;
     if (m < 1 || n < 1 || m < n || lda < m || ldr < n || !A || !R) rc = LATER_ERR_SHAPE;
-    else if (!prec_ok(o.gemm) || !prec_ok(o.gemm_pass2) || o.panel < 0 || o.panel > 2 || o.reorth < 0 ||
-             o.reorth > 3 || o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0 || o.chunk2 < 0 ||
-             (o.chunk2 % 32) != 0)
+    else if (!prec_ok(o.gemm) || !prec_ok(o.gemm_pass2) || !update_ok(o.gemm_update) || o.panel < 0 ||
+             o.panel > 3 || o.reorth < 0 || o.reorth > 3 || o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0 ||
+             o.chunk2 < 0 || (o.chunk2 % 32) != 0 ||
+             ((o.reorth & LATER_REORTH_NODE) && (o.gemm == LATER_GEMM_FP8 || o.gemm_pass2 == LATER_GEMM_FP8)))
         rc = LATER_ERR_OPTION;
     Layout L = layout(m > 0 ? m : 1, n > 0 ? n : 1, o);
     // Everything lives in `work` when it is large enough.  Otherwise the
@@ -344,6 +376,10 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     // enough (and not used for FP32 workspace); otherwise the GEMMs convert
     // their operands in the kernel, with the same result bits.
     const bool with_pieces = work_fits && hwork && L.hpieces > 0 && lhwork >= L.hpieces;
+    // FP8 projections need their staged operands.
+    const bool fp8 = n > o.nmin && (o.gemm == LATER_GEMM_FP8 ||
+                                    ((o.reorth & LATER_REORTH_PASS) && o.gemm_pass2 == LATER_GEMM_FP8));
+    if (!rc && fp8 && !with_pieces) rc = LATER_ERR_WORKSPACE;
     if (!rc) {
         const int64_t abytes = ((int64_t)(n - 1) * lda + m) * 4, rbytes = ((int64_t)(n - 1) * ldr + n) * 4;
         const int64_t wbytes = lwork * 4, hbytes = lhwork * 2;
@@ -404,9 +440,9 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
             hipLaunchKernelGGL(copy_upper_kernel, grid_cols(n, n), dim3(256), 0, r.s, n, R2, (int64_t)n, R2,
                                1.0f);
             r.launches += 2;
-            const int p2 = o.gemm_pass2, pu = update_prec(p2);
-            const int *bexp = scaled(p2) ? r.exps : nullptr;
-            const int aexp = scaled(p2) ? kQExp : 0;
+            const int pu = update_of(o, o.gemm_pass2);
+            const int *bexp = scaled(pu) ? r.exps : nullptr;
+            const int aexp = scaled(pu) ? kQExp : 0;
             const bool pre = r.pcs && n >= kPieceMinN && gemm_pieces(pu) > 0;
             Pieces rt{}, tt{};
             if (pre) {
@@ -414,11 +450,11 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
                 rt = {r.pcs, ld, ld * n};
                 tt = {r.pcs + gemm_pieces(pu) * ld * n, ld, ld * n};
                 rc = gemm_split(r.s, pu, n, n, R2, n, aexp, nullptr, (uint16_t *)rt.p, ld, ld * n, true, &r.launches);
-                if (!rc && scaled(p2))
+                if (!rc && scaled(pu))
                     rc = gemm_split_exp(r.s, pu, n, n, T, n, r.exps, (uint16_t *)tt.p, ld, ld * n, &r.launches);
                 else if (!rc)
                     rc = gemm_split(r.s, pu, n, n, T, n, 0, bexp, (uint16_t *)tt.p, ld, ld * n, false, &r.launches);
-            } else if (scaled(p2)) {
+            } else if (scaled(pu)) {
                 rc = gemm_colexp(r.s, n, n, T, n, r.exps, &r.launches);
             }
             if (!rc)
@@ -484,6 +520,7 @@ LATER_API int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, 
 // piece_ld(n1), m rows) and C column-major (ld piece_ld(n1)).
 static int64_t component_pieces(int m, int n1, int n2, int prec, bool update)
 {
+    if (prec == LATER_GEMM_FP8) return update ? 0 : (fp8_bytes(m, n1) + fp8_bytes(m, n2)) / 2;
     const int np = gemm_pieces(prec);
     if (np == 0) return 0;
     if (!update) return (int64_t)np * piece_ld(m) * ((int64_t)n1 + n2);
@@ -494,7 +531,8 @@ LATER_API int later_gemm_workspace(int m, int n1, int n2, int prec, int chunk, i
                                    int64_t *lhwork)
 {
     if (m < 1 || n1 < 1 || n2 < 1) return LATER_ERR_SHAPE;
-    if (!prec_ok(prec) || chunk < 0 || chunk % 32) return LATER_ERR_OPTION;
+    if (!prec_ok(prec) || chunk < 0 || chunk % 32 || (update && prec == LATER_GEMM_FP8)) return LATER_ERR_OPTION;
+    if (prec == LATER_GEMM_FP8) chunk = 0;
     const int64_t exps = ((int64_t)n2 + 63) / 64 * 64;
     if (lwork) *lwork = update ? exps : exps + gemm_proj_work_floats(m, n1, n2, chunk);
     if (lhwork) *lhwork = component_pieces(m, n1, n2, prec, update != 0);
@@ -507,12 +545,24 @@ LATER_API int later_gemm_proj(laterCtxt ctxt, int m, int n1, int n2, const float
 {
     if (m < 1 || n1 < 1 || n2 < 1 || ldq < m || ldb < m || ldc < n1 || !Q || !B || !C) return LATER_ERR_SHAPE;
     if (!prec_ok(prec) || chunk < 0 || chunk % 32) return LATER_ERR_OPTION;
+    const bool fp8 = prec == LATER_GEMM_FP8;
+    if (fp8) chunk = 0;
     const int64_t exps = ((int64_t)n2 + 63) / 64 * 64;
     const int64_t need = exps + gemm_proj_work_floats(m, n1, n2, chunk);
     if (lwork < need || (!work && need > 0)) return LATER_ERR_WORKSPACE;
     const int64_t hneed = component_pieces(m, n1, n2, prec, false);
-    const bool pre = hwork && hneed > 0 && lhwork >= hneed && (((uintptr_t)hwork) & 15) == 0;
+    const bool pre = hwork && hneed > 0 && lhwork >= hneed && (((uintptr_t)hwork) & 255) == 0;
     hipStream_t s = ctx_stream(ctxt);
+    if (fp8) {
+        // e4m3 operands with per-128-row block scales, staged in hwork.
+        if (!pre) return LATER_ERR_WORKSPACE;
+        const Pieces qh = fp8_pieces(hwork, m, n1), bh = fp8_pieces(hwork + fp8_bytes(m, n1) / 2, m, n2);
+        int rc = gemm_split8(s, m, n1, Q, ldq, (void *)qh.p, nullptr);
+        if (!rc) rc = gemm_split8(s, m, n2, B, ldb, (void *)bh.p, nullptr);
+        if (rc) return rc;
+        return gemm_proj(s, prec, m, n1, n2, Q, ldq, B, ldb, C, ldc, 0, nullptr, 0, work + exps, lwork - exps,
+                         nullptr, &qh, &bh);
+    }
     const int aexp = scaled(prec) ? kQExp : 0;
     const int *bexp = scaled(prec) ? (const int *)work : nullptr;
     Pieces qh{}, bh{};
@@ -539,7 +589,7 @@ LATER_API int later_gemm_update(laterCtxt ctxt, int m, int n1, int n2, const flo
                                 int64_t lhwork)
 {
     if (m < 1 || n1 < 1 || n2 < 1 || ldq < m || ldb < m || ldc < n1 || !Q || !B || !C) return LATER_ERR_SHAPE;
-    if (!prec_ok(prec)) return LATER_ERR_OPTION;
+    if (!prec_ok(prec) || prec == LATER_GEMM_FP8) return LATER_ERR_OPTION;
     const int64_t exps = ((int64_t)n2 + 63) / 64 * 64;
     if (scaled(prec) && (lwork < exps || !work)) return LATER_ERR_WORKSPACE;
     const int64_t hneed = component_pieces(m, n1, n2, prec, true);

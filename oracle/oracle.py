@@ -180,3 +180,48 @@ def rel_err(X, Y) -> float:
     Y = np.asarray(Y, dtype=np.float64)
     den = np.linalg.norm(Y)
     return float(np.linalg.norm(X - Y) / (den if den > 0 else 1.0))
+
+
+# ------------------------------------------------------------------ FP8
+# The e4m3 operand quantisation of the FP8 projections (no counterpart in the
+# reference, whose only cast is the FP16 one of util/util.cu:24-32): every
+# column is cut into blocks of 128 rows; a block with largest magnitude mx is
+# scaled by 2^e with e the largest integer keeping mx 2^e <= 448 (e4m3's
+# largest finite value; 448 = 0.875 * 2^9), rounded to the nearest e4m3
+# value (ties to even), and carries the E8M0 scale 127 - e.
+
+E4M3_MAX = 448.0
+
+
+def e4m3_round(y: np.ndarray) -> np.ndarray:
+    """Round |y| <= 448 to the nearest OCP e4m3 value, ties to even: 3
+    fraction bits, exponents -6 .. 8, subnormal spacing 2^-9."""
+    y = np.asarray(y, dtype=np.float64)
+    a = np.abs(y)
+    with np.errstate(divide="ignore"):
+        ex = np.floor(np.log2(np.where(a > 0, a, 1.0)))
+    ex = np.maximum(ex, -6.0)
+    ulp = np.exp2(ex - 3.0)
+    return np.round(y / ulp) * ulp  # np.round: half to even
+
+
+def fp8_block_exponents(X: np.ndarray) -> np.ndarray:
+    """e per (128-row block, column): mx 2^e in (224, 448] (0 for a zero block)."""
+    X = np.asarray(X, dtype=np.float64)
+    K, n = X.shape
+    nb = (K + 127) // 128
+    e = np.zeros((nb, n), dtype=np.int64)
+    for b in range(nb):
+        mx = np.abs(X[128 * b:128 * (b + 1)]).max(axis=0)
+        f, fe = np.frexp(mx)  # mx = f 2^fe, f in [0.5, 1)
+        eb = np.where(f <= 0.875, 9 - fe, 8 - fe)
+        e[b] = np.where(mx > 0, np.clip(eb, -126, 126), 0)
+    return e
+
+
+def fp8_quantize(X: np.ndarray) -> np.ndarray:
+    """The FP64 values the FP8 product multiplies: e4m3(x 2^e) 2^-e."""
+    X = np.asarray(X, dtype=np.float64)
+    e = fp8_block_exponents(X)
+    sc = np.repeat(np.exp2(e.astype(np.float64)), 128, axis=0)[: X.shape[0]]
+    return e4m3_round(X * sc) / sc

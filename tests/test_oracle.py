@@ -207,3 +207,75 @@ def test_planted_mistakes_are_caught(oracle):
     for bug in ("drop", "transpose"):
         Qb, Rb = rec(A, bug)
         assert oracle.rel_err(Rb, R64) > 1e-3, bug
+
+
+# ---------------------------------------------------------------- FP8 quantisation
+def _e4m3_codes():
+    vals = []
+    for c in range(256):
+        s, e, m = c >> 7, (c >> 3) & 15, c & 7
+        if e == 15 and m == 7:
+            continue  # NaN
+        v = (m / 8.0) * 2.0 ** -6 if e == 0 else (1 + m / 8.0) * 2.0 ** (e - 7)
+        vals.append(-v if s else v)
+    return np.unique(np.array(vals))
+
+
+def test_e4m3_round_matches_torch_and_brute_force(oracle):
+    import torch
+    rng = np.random.default_rng(3)
+    y = (rng.standard_normal(400000) * np.exp2(rng.integers(-14, 9, 400000))).clip(-448, 448).astype(np.float32)
+    codes = _e4m3_codes()
+    # exact ties between neighbouring codes (round half to even)
+    mids = ((codes[:-1] + codes[1:]) / 2).astype(np.float32)
+    y = np.concatenate([y, mids, codes.astype(np.float32)])
+    mine = oracle.e4m3_round(y.astype(np.float64))
+    t = torch.from_numpy(y).to(torch.float8_e4m3fn).float().numpy().astype(np.float64)
+    assert np.array_equal(mine, t)
+    # brute force: the nearest code, ties to the code with an even last bit
+    idx = np.searchsorted(codes, y.astype(np.float64))
+    lo = codes[np.clip(idx - 1, 0, len(codes) - 1)]
+    hi = codes[np.clip(idx, 0, len(codes) - 1)]
+    d_lo, d_hi = np.abs(y - lo), np.abs(hi - y)
+    near = np.where(d_lo < d_hi, lo, hi)
+    tie = d_lo == d_hi
+    assert np.array_equal(mine[~tie], near[~tie])
+    assert np.all(np.isin(mine[tie], codes))
+
+
+def test_fp8_block_exponents(oracle):
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((1000, 7)) * np.exp2(rng.integers(-40, 40, (1, 7)))
+    X[:, 3] = 0.0
+    X[128:256, 5] = 448.0  # exactly the top: e = 0
+    e = oracle.fp8_block_exponents(X)
+    assert e.shape == (8, 7)
+    for b in range(8):
+        mx = np.abs(X[128 * b:128 * (b + 1)]).max(axis=0)
+        s = mx * np.exp2(e[b].astype(float))
+        nz = mx > 0
+        assert np.all((s[nz] > 224) & (s[nz] <= 448))
+    assert np.all(e[:, 3] == 0) and e[1, 5] == 0
+    Xq = oracle.fp8_quantize(X)
+    # every value moves by at most half an e4m3 spacing of its block's top binade
+    for b in range(8):
+        blk = slice(128 * b, 128 * (b + 1))
+        mx = np.abs(X[blk]).max(axis=0)
+        assert np.all(np.abs(Xq[blk] - X[blk]) <= mx * 2.0 ** -4 + 1e-300)
+    # the scaled values are e4m3 codes
+    codes = _e4m3_codes()
+    sc = np.repeat(np.exp2(e.astype(float)), 128, axis=0)[:1000]
+    assert np.all(np.isin(Xq * sc, codes))
+    # planted mistakes: a scale 2^12 too small (values fall into the
+    # subnormals) changes the result; one binade too large for blocks whose
+    # top lies in (448, 512) 2^-e leaves the code set
+    Y = rng.standard_normal((128, 50))
+    ey = oracle.fp8_block_exponents(Y)[0]
+    small = oracle.e4m3_round(Y * np.exp2(ey - 12.0)) / np.exp2(ey - 12.0)
+    assert not np.array_equal(small, oracle.fp8_quantize(Y))
+    mx = np.abs(Y).max(axis=0)
+    f, fe = np.frexp(mx)
+    big = f > 0.875
+    assert big.any()
+    top = mx[big] * np.exp2(9.0 - fe[big])
+    assert not np.any(np.isin(oracle.e4m3_round(top), codes))
