@@ -154,10 +154,14 @@ def single_config(later, torch, name, m, n, kind, forms, steps=3):
     return out
 
 
-def batched_config(later, torch, count=4096, form="mixed", steps=3):
+def batched_config(later, torch, count=4096, steps=3):
     """4096 independent matrices, m log-uniform in [64, 4096], n uniform in
-    [16, min(m, 256)], entries 2u - 1 (Philox, seed 0, matrix index i), all
-    factored by one graph replay of a batched plan."""
+    [16, min(m, 256)], entries 2u - 1 (Philox, seed 0, matrix index i).  The
+    headline engine is the persistent kernel (one launch; a workgroup per
+    matrix from a device work queue; per-bucket autotuned variants), in the
+    FP32-product form with reorthogonalisation ("mixed" options) and the
+    BF16 one; the grouped-launch graph of the shared recursion tree is timed
+    beside it."""
     import numpy as np
     rng = np.random.default_rng(0)
     m = np.rint(np.exp(rng.uniform(np.log(64), np.log(4096), count))).astype(int)
@@ -165,33 +169,51 @@ def batched_config(later, torch, count=4096, form="mixed", steps=3):
     As = [later.generate_uniform(int(mi), int(ni), seed=0, mat=i) for i, (mi, ni) in enumerate(zip(m, n))]
     A0 = [a.clone() for a in As]
     Rs = [later.colmajor(int(ni), int(ni)) for ni in n]
-    t0 = time.time()
-    plan = later.BatchPlan(As, Rs, later.opts(form))
-    build_s = time.time() - t0
-    s = torch.cuda.current_stream()
-    times = []
-    for it in range(steps + 1):
-        for a, a0 in zip(As, A0):
-            a.copy_(a0)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record(s)
-        plan.execute()
-        e1.record(s)
-        torch.cuda.synchronize()
-        if it:
-            times.append(e0.elapsed_time(e1))
-    ms = sum(times) / len(times)
     flops = float(np.sum(2.0 * m * n * n))
     nbytes = float(np.sum(8.0 * m * n))
-    worst = 0.0
-    for i in range(0, count, 512):  # spot checks
-        worst = max(worst, later.check_orth(As[i]))
-    out = {"count": count, "form": form, "ms": ms, "tflops_mgs": flops / ms / 1e9, "gbps": nbytes / ms / 1e6,
-           "launches": plan.launches, "plan_build_s": build_s, "orth_max_sampled": worst,
-           "shapes": "m log-uniform [64, 4096], n uniform [16, min(m, 256)], seed 0"}
-    log(f"[bench] C4 batched {count} matrices ({form}): {ms:.2f} ms  {out['tflops_mgs']:.2f} TFLOP/s(2mn^2)  "
-        f"{out['gbps']:.0f} GB/s  launches={plan.launches}  build {build_s:.1f} s  orth<={worst:.1e}")
+    s = torch.cuda.current_stream()
+
+    def timed(plan):
+        times = []
+        for it in range(steps + 1):
+            for a, a0 in zip(As, A0):
+                a.copy_(a0)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(s)
+            plan.execute()
+            e1.record(s)
+            torch.cuda.synchronize()
+            if it:
+                times.append(e0.elapsed_time(e1))
+        worst = 0.0
+        for i in range(0, count, 256):  # spot checks
+            worst = max(worst, later.check_orth(As[i]))
+        return sum(times) / len(times), worst
+
+    out = {"count": count, "shapes": "m log-uniform [64, 4096], n uniform [16, min(m, 256)], seed 0",
+           "flops_2mn2": flops, "bytes_8mn": nbytes}
+    forms = {"persistent_fp32_reorth": later.opts("mixed"),
+             "persistent_bf16_reorth": later.opts("reference", gemm=later.GEMM_BF16, reorth=later.REORTH_PASS)}
+    for name, o in forms.items():
+        t0 = time.time()
+        plan = later.BatchWg(As, Rs, o)
+        build_s = time.time() - t0
+        ms, worst = timed(plan)
+        bad = int(plan.status.abs().sum())
+        out[name] = {"ms": ms, "tflops_mgs": flops / ms / 1e9, "gbps": nbytes / ms / 1e6, "launches": 1,
+                     "plan_build_s": build_s, "orth_max_sampled": worst, "breakdowns": bad}
+        log(f"[bench] C4 batched {count} matrices ({name}): {ms:.2f} ms  {flops / ms / 1e9:.2f} TFLOP/s(2mn^2)  "
+            f"{nbytes / ms / 1e6:.0f} GB/s  1 launch  orth<={worst:.1e}")
+        plan.close()
+    t0 = time.time()
+    plan = later.BatchPlan(As, Rs, later.opts("mixed"))
+    build_s = time.time() - t0
+    ms, worst = timed(plan)
+    out["grouped_mixed"] = {"ms": ms, "tflops_mgs": flops / ms / 1e9, "gbps": nbytes / ms / 1e6,
+                            "launches": plan.launches, "plan_build_s": build_s, "orth_max_sampled": worst}
+    log(f"[bench] C4 batched {count} matrices (grouped launches, mixed): {ms:.2f} ms  "
+        f"{flops / ms / 1e9:.2f} TFLOP/s(2mn^2)  launches={plan.launches}  orth<={worst:.1e}")
     plan.close()
     del As, A0, Rs
     torch.cuda.empty_cache()

@@ -200,6 +200,32 @@ int later_batch_plan_execute(void *plan, void *stream);
 int later_batch_plan_launches(void *plan);
 int later_batch_plan_destroy(void *plan);
 
+/* Batched QR in ONE persistent launch (no counterpart in the reference).
+ * later_batch_wg_create builds the task table (one task per matrix, ordered
+ * by size bucket and decreasing cost, largest first) and the device work
+ * queue, allocating them once with hipMalloc; later_batch_wg_execute
+ * enqueues one kernel on `stream` whose workgroups each take whole matrices
+ * from the queue (an atomic queue head; the last workgroup to finish resets
+ * it, so the plan can be executed again).  Per matrix: left-looking block
+ * Gram-Schmidt, each panel projected out of all earlier columns of Q twice
+ * when opts->reorth != 0 (once otherwise), then factored by Cholesky QR
+ * twice with FP64 Gram matrices; products on MFMA in FP32
+ * (opts->gemm F32 and the split forms), BF16 or FP16 (opts->gemm BF16 /
+ * F16).  A is overwritten by Q and R (n x n, zero below the diagonal).
+ * status (optional, count ints): bit 0 set for a matrix whose panel had a
+ * bad pivot.  tuning (optional): LATER_BATCH_BUCKETS variant ids, one per
+ * size bucket later_batch_wg_bucket(m, n) = 7 (min(ceil(n/32), 8) - 1) +
+ * clamp(floor(log2 m) - 6, 0, 6); variant v uses 16 << (v & 1) columns per
+ * panel and blocks of 64 << (v >> 1) earlier Q columns; NULL (or -1 entries)
+ * selects the built-in table written by later_amd/autotune.py. */
+#define LATER_BATCH_BUCKETS 56
+int later_batch_wg_create(int count, const int *m, const int *n, float *const *A, const int *lda,
+                          float *const *R, const int *ldr, int *status, const later_opts *opts,
+                          const int *tuning, void **plan);
+int later_batch_wg_execute(void *plan, void *stream);
+int later_batch_wg_destroy(void *plan);
+int later_batch_wg_bucket(int m, int n);
+
 /* Synchronises ctxt.stream and reads the device status word that the last
  * later_rgsqrf_ex / later_caqr_panel call on this `work` left: bit 0 set
  * means a zero or non-finite column norm (LATER_ERR_BREAKDOWN); the

@@ -110,6 +110,12 @@ _SIGNATURES = {
     "later_batch_plan_execute": (_i, [_vp, _vp]),
     "later_batch_plan_launches": (_i, [_vp]),
     "later_batch_plan_destroy": (_i, [_vp]),
+    "later_batch_wg_create": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_vp),
+                                   ctypes.POINTER(_i), ctypes.POINTER(_vp), ctypes.POINTER(_i), _vp,
+                                   ctypes.POINTER(later_opts), ctypes.POINTER(_i), ctypes.POINTER(_vp)]),
+    "later_batch_wg_execute": (_i, [_vp, _vp]),
+    "later_batch_wg_destroy": (_i, [_vp]),
+    "later_batch_wg_bucket": (_i, [_i, _i]),
     "later_prof_begin": (None, []),
     "later_prof_end": (_i, [ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
 }
@@ -278,6 +284,56 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+BATCH_BUCKETS = 56
+
+
+class BatchWg:
+    """Batched QR of the column-major tensors As (in place) into Rs in one
+    persistent kernel launch over a device work queue (later_batch_wg_*).
+    tuning: None (built-in table) or BATCH_BUCKETS variant ids."""
+
+    def __init__(self, As, Rs, o: later_opts | None = None, tuning=None, status: bool = True):
+        torch = _torch()
+        cnt = len(As)
+        I = _i * cnt
+        P = _vp * cnt
+        self.m = I(*[a.shape[0] for a in As])
+        self.n = I(*[a.shape[1] for a in As])
+        self.lda = I(*[_ld(a) for a in As])
+        self.ldr = I(*[_ld(r) for r in Rs])
+        self.A = P(*[a.data_ptr() for a in As])
+        self.R = P(*[r.data_ptr() for r in Rs])
+        self.status = torch.zeros(cnt, device=As[0].device, dtype=torch.int32) if status else None
+        self.keep = (As, Rs)
+        tun = (_i * BATCH_BUCKETS)(*[int(v) for v in tuning]) if tuning is not None else None
+        h = _vp()
+        _check(lib().later_batch_wg_create(cnt, self.m, self.n, self.A, self.lda, self.R, self.ldr,
+                                           self.status.data_ptr() if status else None,
+                                           ctypes.byref(o) if o is not None else None, tun, ctypes.byref(h)),
+               "later_batch_wg_create")
+        self.h = h
+
+    def execute(self, stream=None):
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _check(lib().later_batch_wg_execute(self.h, ctypes.c_void_p(s.cuda_stream)), "later_batch_wg_execute")
+
+    def close(self):
+        if self.h:
+            lib().later_batch_wg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_bucket(m: int, n: int) -> int:
+    return int(lib().later_batch_wg_bucket(m, n))
 
 
 class BatchPlan:
