@@ -46,7 +46,7 @@ def time_form(later, torch, A0, form, steps, warmup, graph=True):
     outside the timed events).  With graph=True the factorisation is one
     replay of a graph-captured plan; the first eager call is timed too."""
     m, n = A0.shape
-    o = later.opts(form)
+    o = later.opts(form) if isinstance(form, str) else form
     work = later.workspace(m, n, o)
     hwork = later.hworkspace(m, n, o)  # 16-bit operand staging (None for the FP32 form)
     A = later.colmajor(m, n)
@@ -154,6 +154,82 @@ def single_config(later, torch, name, m, n, kind, forms, steps=3):
     return out
 
 
+def blas_ceiling(torch, M=16384, N=8192, K=8192, reps=5):
+    """The vendor library's dense GEMM rate on this GPU (torch.matmul, i.e.
+    hipBLASLt) at the root update's shape, FP16 and FP32: the practical
+    ceilings the node GEMMs are compared with."""
+    out = {}
+    for name, dt in (("fp16", torch.float16), ("fp32", torch.float32)):
+        a = torch.randn(M, K, device="cuda", dtype=dt)
+        b = torch.randn(K, N, device="cuda", dtype=dt)
+        for _ in range(2):
+            c = a @ b
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            c = a @ b
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"ms": ms, "tflops": 2.0 * M * N * K / ms / 1e9, "shape": [M, N, K]}
+        del a, b, c
+    torch.cuda.empty_cache()
+    log(f"[bench] vendor GEMM ceiling {M}x{N}x{K}: fp16 {out['fp16']['tflops']:.0f} TFLOP/s, "
+        f"fp32 {out['fp32']['tflops']:.0f} TFLOP/s")
+    return out
+
+
+def square_variants(later, torch, A0, steps=2):
+    """The 16384^2 square workload's other forms: FP16 and BF16 node GEMMs
+    with and without the second pass (Cholesky QR leaves when repeated), on
+    the same N(0,1) input; and the graded input A = U diag(s) V^T (U, V the
+    Q factors of two Gaussian matrices, s geometric from 1 to 1e-4) in the
+    mixed and plain-FP32 forms."""
+    m, n = A0.shape
+    out = {}
+    variants = {
+        "fp16": later.opts("reference"),
+        "fp16_reorth": later.opts("reference", reorth=later.REORTH_PASS, panel=later.PANEL_CHOLQR1, chunk=256,
+                                  chunk2=32),
+        "bf16": later.opts("reference", gemm=later.GEMM_BF16, gemm_pass2=later.GEMM_BF16),
+        "bf16_reorth": later.opts("reference", gemm=later.GEMM_BF16, gemm_pass2=later.GEMM_BF16,
+                                  reorth=later.REORTH_PASS, panel=later.PANEL_CHOLQR1),
+    }
+
+    def run(name, A, o):
+        times, eager_ms, Q, R, _, _ = time_form(later, torch, A, o, steps, 1)
+        ms = sum(times) / len(times)
+        e = {"ms": ms, "tflops_ref": later.flops_ref(m, n) / ms / 1e9, "orth": later.check_orth(Q),
+             "backward": later.check_backward(A, Q, R)}
+        e["gates_met"] = e["orth"] <= 1e-5 and e["backward"] <= 1e-6
+        log(f"[bench] C2 {name}: {ms:.2f} ms  {e['tflops_ref']:.1f} TFLOP/s(ref)  orth={e['orth']:.2e} "
+            f"bwd={e['backward']:.2e}")
+        del Q, R
+        torch.cuda.empty_cache()
+        return e
+
+    for name, o in variants.items():
+        out[name] = run(name, A0, o)
+    # Graded input.
+    U = later.generate_normal(m, n, 1, 0)
+    later.rgsqrf(U, o=later.opts("mixed"))
+    V = later.generate_normal(n, n, 2, 0)
+    later.rgsqrf(V, o=later.opts("mixed"))
+    s = torch.pow(torch.tensor(1e-4, dtype=torch.float64, device="cuda"),
+                  torch.arange(n, device="cuda", dtype=torch.float64) / (n - 1)).float()
+    G = (U * s[None, :]) @ V.t()
+    Ag = later.colmajor(m, n)
+    Ag.copy_(G)
+    del U, V, G
+    torch.cuda.empty_cache()
+    for form in ("mixed", "fp32"):
+        out["graded_" + form] = run("graded (cond 1e4) " + form, Ag, later.opts(form))
+    del Ag
+    torch.cuda.empty_cache()
+    return out
+
+
 def batched_config(later, torch, count=4096, steps=3):
     """4096 independent matrices, m log-uniform in [64, 4096], n uniform in
     [16, min(m, 256)], entries 2u - 1 (Philox, seed 0, matrix index i).  The
@@ -220,13 +296,16 @@ def batched_config(later, torch, count=4096, steps=3):
     return out
 
 
-def serving_config(later, torch, count=64, rate=100.0, slots=8, budget=16 << 30):
+def serving_config(later, torch, count=64, rates=(100.0, 1e9), slots=8, budget=16 << 30):
     """Least-squares serving: requests with m log-uniform in [128, 32768],
     n log-uniform in [16, 1024] (n <= m), condition number log-uniform in
     [1, 1e6] (A = U diag(s) V^T, s_i = cond^(-i/(n-1)), U and V from FP64 QR
     of Gaussians), k uniform in [1, 16] right-hand sides, b = A x + 1e-3 noise,
-    arriving as a Poisson process of `rate` requests/s.  Latency is the
-    server's own submit-to-done time; x is checked against FP64 least squares."""
+    arriving as a Poisson process of each of `rates` requests/s (1e9: all at
+    once, the saturation throughput).  Latency is the server's own
+    submit-to-done time; every x is checked against FP64 least squares and
+    against the first-order LS perturbation bound for a backward error of
+    1e-6, kappa eps + kappa^2 eps ||r|| / (||A|| ||x||)."""
     import numpy as np
     rng = np.random.default_rng(1)
     g = torch.Generator(device="cuda").manual_seed(1)
@@ -244,51 +323,71 @@ def serving_config(later, torch, count=64, rate=100.0, slots=8, budget=16 << 30)
         b64 = A64 @ x + 1e-3 * torch.randn(m, k, device="cuda", dtype=torch.float64, generator=g)
         A = A64.float().t().contiguous().t()
         B = b64.float().t().contiguous().t()
-        x64 = torch.linalg.lstsq(A.double(), B.double()).solution
-        probs.append((A, B, x64, m, n, k, cond))
-        del U, V, A64, b64
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+        Ad, Bd = A.double(), B.double()
+        x64 = torch.linalg.lstsq(Ad, Bd).solution
+        svals = torch.linalg.svdvals(Ad)
+        kap = float(svals[0] / svals[-1])
+        eps = 1e-6
+        rn = float(torch.linalg.norm(Bd - Ad @ x64))
+        bound = kap * eps + kap * kap * eps * rn / (float(svals[0]) * float(torch.linalg.norm(x64)))
+        probs.append((A, B, x64, m, n, k, cond, bound))
+        del U, V, A64, b64, Ad, Bd
     torch.cuda.synchronize()
-    srv = later.Server(slots=slots, hbm_budget_bytes=budget)
-    ids, lat, errs, done = [None] * count, [], [], 0
-    t0 = time.perf_counter()
-    nxt = 0
-    pending = set()
-    while done < count:
-        now = time.perf_counter() - t0
-        while nxt < count and arrivals[nxt] <= now:
-            A, B = probs[nxt][0], probs[nxt][1]
-            ids[nxt] = srv.submit(A, B)
-            pending.add(nxt)
-            nxt += 1
-        srv.poll()
-        for i in sorted(pending):
-            r = srv.result(ids[i])
-            if r["state"] in (3, 4):
-                A, B, x64, m, n, k, cond = probs[i]
-                if r["state"] == 3:
-                    X, _ = srv.fetch(ids[i], m, n, k)
-                    errs.append(float(torch.linalg.norm(X.double() - x64) / torch.linalg.norm(x64)))
-                    lat.append(r["latency_s"])
-                srv.release(ids[i])
-                pending.discard(i)
-                done += 1
-        if time.perf_counter() - t0 > 120:
-            break
-    wall = time.perf_counter() - t0
-    stats = srv.stats()
-    srv.close()
-    lat_ms = np.array(lat) * 1e3
-    out = {"requests": count, "completed": len(lat), "rate_per_s": rate, "slots": slots,
-           "p50_ms": float(np.percentile(lat_ms, 50)) if len(lat) else None,
-           "p99_ms": float(np.percentile(lat_ms, 99)) if len(lat) else None,
-           "solves_per_s": len(lat) / wall, "x_rel_err_median": float(np.median(errs)) if errs else None,
-           "x_rel_err_max": float(np.max(errs)) if errs else None, "peak_pool_bytes": stats["peak_bytes"],
-           "workload": "m log-uniform [128, 32768], n log-uniform [16, 1024], cond log-uniform [1, 1e6], "
-                       "k in [1, 16], Poisson arrivals"}
-    log(f"[bench] C5 serving {count} requests at {rate}/s: p50 {out['p50_ms']:.1f} ms  p99 {out['p99_ms']:.1f} ms  "
-        f"{out['solves_per_s']:.1f} solves/s  |x-x64|/|x64| median {out['x_rel_err_median']:.1e} "
-        f"max {out['x_rel_err_max']:.1e}")
+    runs = {}
+    for rate in rates:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+        srv = later.Server(slots=slots, hbm_budget_bytes=budget)
+        ids, lat, errs, ratio, done = [None] * count, [], [], [], 0
+        t0 = time.perf_counter()
+        nxt = 0
+        pending = set()
+        while done < count:
+            now = time.perf_counter() - t0
+            while nxt < count and arrivals[nxt] <= now:
+                A, B = probs[nxt][0], probs[nxt][1]
+                ids[nxt] = srv.submit(A, B)
+                pending.add(nxt)
+                nxt += 1
+            srv.poll()
+            for i in sorted(pending):
+                r = srv.result(ids[i])
+                if r["state"] in (3, 4):
+                    A, B, x64, m, n, k, cond, bound = probs[i]
+                    if r["state"] == 3:
+                        X, _ = srv.fetch(ids[i], m, n, k)
+                        e = float(torch.linalg.norm(X.double() - x64) / torch.linalg.norm(x64))
+                        errs.append(e)
+                        ratio.append(e / (bound + 1e-7))
+                        lat.append(r["latency_s"])
+                    srv.release(ids[i])
+                    pending.discard(i)
+                    done += 1
+            if time.perf_counter() - t0 > 120:
+                break
+        wall = time.perf_counter() - t0
+        stats = srv.stats()
+        srv.close()
+        lat_ms = np.array(lat) * 1e3
+        key = "saturation" if rate >= 1e8 else f"rate_{rate:g}"
+        runs[key] = {"requests": count, "completed": len(lat), "rate_per_s": None if rate >= 1e8 else rate,
+                     "slots": slots,
+                     "p50_ms": float(np.percentile(lat_ms, 50)) if len(lat) else None,
+                     "p99_ms": float(np.percentile(lat_ms, 99)) if len(lat) else None,
+                     "solves_per_s": len(lat) / wall, "wall_s": wall,
+                     "x_rel_err_median": float(np.median(errs)) if errs else None,
+                     "x_rel_err_max": float(np.max(errs)) if errs else None,
+                     "err_over_ls_bound_max": float(np.max(ratio)) if ratio else None,
+                     "within_10x_ls_bound": int(np.sum(np.array(ratio) < 10.0)),
+                     "peak_pool_bytes": stats["peak_bytes"]}
+        o = runs[key]
+        log(f"[bench] C5 serving {count} requests, {'all at once' if rate >= 1e8 else f'{rate:g}/s'}: "
+            f"p50 {o['p50_ms']:.1f} ms  p99 {o['p99_ms']:.1f} ms  {o['solves_per_s']:.1f} solves/s  "
+            f"|x-x64|/|x64| median {o['x_rel_err_median']:.1e} max {o['x_rel_err_max']:.1e}  "
+            f"{o['within_10x_ls_bound']}/{len(lat)} within 10x the LS bound")
+    out = dict(runs["rate_100"]) if "rate_100" in runs else {}
+    out["runs"] = runs
+    out["workload"] = ("m log-uniform [128, 32768], n log-uniform [16, 1024], cond log-uniform [1, 1e6], "
+                       "k in [1, 16], Poisson arrivals")
     del probs
     torch.cuda.empty_cache()
     return out
@@ -380,8 +479,8 @@ def main():
                     help="three 1 GiB device-to-device copies (the HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this QR path runs on one GPU; multi-GPU row sharding is described "
-                                   "in DESIGN.md and not built (single-GPU pool)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this QR path runs on one GPU; multi-GPU row sharding is not built",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch
@@ -453,6 +552,8 @@ def main():
     extra = None
     if not args.no_extra:
         extra = {
+            "vendor_gemm_ceiling": blas_ceiling(torch),
+            "C2": square_variants(later, torch, A0),
             "C1": single_config(later, torch, "C1", 8192, 1024, "uniform", ("reference", "mixed")),
             "C3a": single_config(later, torch, "C3a", 1048576, 128, "normal", ("reference", "mixed", "fp8", "bf16")),
             "C3b": single_config(later, torch, "C3b", 262144, 256, "normal", ("reference", "mixed", "fp8", "bf16")),

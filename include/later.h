@@ -52,7 +52,8 @@ enum {
     LATER_ERR_DEVICE = 4,     /* HIP runtime error */
     LATER_ERR_BREAKDOWN = 5,  /* zero or non-finite column norm (rank deficiency, NaN/Inf) */
     LATER_ERR_ALIAS = 6,      /* R / work / hwork overlap */
-    LATER_ERR_BUDGET = 7      /* serving: request exceeds the HBM budget */
+    LATER_ERR_BUDGET = 7,     /* serving: request exceeds the HBM budget */
+    LATER_ERR_CANCELLED = 8   /* serving: request cancelled */
 };
 
 /* GEMM operand precision of the recursion nodes. */
@@ -286,7 +287,12 @@ int later_gels(laterCtxt ctxt, int m, int n, int k, float *A, int lda, float *R,
  * request without blocking.  Results (Q, R, X) stay resident in the pool
  * until later_server_release.  States: 0 waiting, 1 QR, 2 solve, 3 done,
  * 4 failed, 5 robust re-factorisation (after an orthogonality sketch above
- * 1e-5).  Calls on one server are serialised by an internal mutex. */
+ * 1e-5), 6 cancelled.  later_server_cancel removes a waiting request from
+ * the queue at once; a running one finishes the stage already enqueued on
+ * its stream and is then retired without further launches (state 6 at the
+ * next poll); a finished one cannot be cancelled.  Cancelled requests are
+ * released like finished ones.  Calls on one server are serialised by an
+ * internal mutex. */
 int later_server_create(int slots, int64_t hbm_budget_bytes, const later_opts *qr_opts, void **srv);
 int later_server_submit(void *srv, int m, int n, int k, const float *A, int lda, const float *B, int ldb,
                         uint64_t *id);
@@ -295,6 +301,7 @@ int later_server_result(void *srv, uint64_t id, int *state, int *status, const f
                         const float **Q, const float **R, double *latency_s, double *queue_s,
                         double *cond, int *iters);
 int later_server_fetch(void *srv, uint64_t id, float *X, int ldx, float *R, int ldr, float *Q, int ldq);
+int later_server_cancel(void *srv, uint64_t id);
 int later_server_release(void *srv, uint64_t id);
 int later_server_stats(void *srv, int64_t *pool_bytes, int64_t *used_bytes, int64_t *peak_bytes,
                        int *completed, int *rejected);

@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblater_amd.so")
 
 # Status codes (include/later.h)
-OK, ERR_SHAPE, ERR_WORKSPACE, ERR_OPTION, ERR_DEVICE, ERR_BREAKDOWN, ERR_ALIAS, ERR_BUDGET = range(8)
+OK, ERR_SHAPE, ERR_WORKSPACE, ERR_OPTION, ERR_DEVICE, ERR_BREAKDOWN, ERR_ALIAS, ERR_BUDGET, ERR_CANCELLED = range(9)
 GEMM_F32, GEMM_F16, GEMM_BF16, GEMM_F16X3, GEMM_F16X6, GEMM_FP8 = range(6)
 PANEL_MGS, PANEL_MGS64, PANEL_CHOLQR2, PANEL_CHOLQR1 = range(4)
 REORTH_NONE, REORTH_NODE, REORTH_PASS = range(3)
@@ -28,7 +28,7 @@ REORTH_NONE, REORTH_NODE, REORTH_PASS = range(3)
 _STATUS_NAMES = {
     ERR_SHAPE: "bad shape", ERR_WORKSPACE: "workspace too small", ERR_OPTION: "bad option",
     ERR_DEVICE: "HIP runtime error", ERR_BREAKDOWN: "breakdown (zero or non-finite column norm)",
-    ERR_ALIAS: "aliased buffers", ERR_BUDGET: "exceeds HBM budget",
+    ERR_ALIAS: "aliased buffers", ERR_BUDGET: "exceeds HBM budget", ERR_CANCELLED: "cancelled",
 }
 
 
@@ -93,6 +93,7 @@ _SIGNATURES = {
                                  ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                  ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_i)]),
     "later_server_release": (_i, [_vp, ctypes.c_uint64]),
+    "later_server_cancel": (_i, [_vp, ctypes.c_uint64]),
     "later_server_fetch": (_i, [_vp, ctypes.c_uint64, _vp, _i, _vp, _i, _vp, _i]),
     "later_server_stats": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                 ctypes.POINTER(_i), ctypes.POINTER(_i)]),
@@ -564,6 +565,9 @@ class Server:
 
     def release(self, rid: int):
         _check(lib().later_server_release(self.h, rid), "later_server_release")
+
+    def cancel(self, rid: int):
+        _check(lib().later_server_cancel(self.h, rid), "later_server_cancel")
 
     def stats(self) -> dict:
         pb, ub, pk = _i64(), _i64(), _i64()

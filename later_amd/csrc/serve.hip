@@ -152,7 +152,8 @@ __global__ __launch_bounds__(256) void sketch_sum_kernel(int n, const double *d2
     if (threadIdx.x == 0) *eta = sqrt(red[0][0] / red[1][0]);
 }
 
-enum State { WAITING = 0, RUNNING_QR = 1, RUNNING_SOLVE = 2, DONE = 3, FAILED = 4, RUNNING_QR2 = 5 };
+enum State { WAITING = 0, RUNNING_QR = 1, RUNNING_SOLVE = 2, DONE = 3, FAILED = 4, RUNNING_QR2 = 5,
+             CANCELLED = 6 };
 
 struct Req {
     uint64_t id;
@@ -173,6 +174,7 @@ struct Req {
     int iters = 0;
     int state = WAITING;
     int status = LATER_OK;
+    bool cancel = false;          // cancellation requested while running
     double t_submit = 0, t_admit = 0, t_done = 0;
 };
 
@@ -368,7 +370,10 @@ void progress(Server &S)
         hipError_t q = hipEventQuery(r->ev);
         if (q == hipErrorNotReady) continue;
         if (q != hipSuccess) { retire(S, r, FAILED, LATER_ERR_DEVICE); continue; }
-        if (r->state == RUNNING_QR && !(r->cond_host[1] <= S.eta_max)) {
+        if (r->cancel) {
+            // The enqueued stage has finished; nothing further is launched.
+            retire(S, r, CANCELLED, LATER_ERR_CANCELLED);
+        } else if (r->state == RUNNING_QR && !(r->cond_host[1] <= S.eta_max)) {
             // Q is not orthogonal enough for the solve: refactor robustly.
             r->state = RUNNING_QR2;
             int rc = launch_refactor(S, r);
@@ -487,7 +492,7 @@ LATER_API int later_server_result(void *srv, uint64_t id, int *state, int *statu
     if (X) *X = fin ? r->X : nullptr;
     if (Q) *Q = fin ? r->A : nullptr;
     if (R) *R = fin ? r->R : nullptr;
-    const bool ended = (r->state == DONE || r->state == FAILED);
+    const bool ended = (r->state == DONE || r->state == FAILED || r->state == CANCELLED);
     if (latency_s) *latency_s = ended ? r->t_done - r->t_submit : -1.0;
     if (queue_s) *queue_s = (r->t_admit > 0) ? r->t_admit - r->t_submit : -1.0;
     if (cond) *cond = r->cond;
@@ -516,6 +521,35 @@ LATER_API int later_server_fetch(void *srv, uint64_t id, float *X, int ldx, floa
     return LATER_OK;
 }
 
+// Cancels a request: a waiting one leaves the queue at once (state
+// CANCELLED); a running one finishes the stage already on its stream, is not
+// advanced further and becomes CANCELLED at the next poll.  Either way its
+// memory returns to the pool at later_server_release.  A finished request
+// cannot be cancelled (LATER_ERR_SHAPE).
+LATER_API int later_server_cancel(void *srv, uint64_t id)
+{
+    Server *S = (Server *)srv;
+    if (!S) return LATER_ERR_SHAPE;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return LATER_ERR_SHAPE;
+    Req *r = it->second;
+    if (r->state == WAITING) {
+        S->waiting.erase(std::find(S->waiting.begin(), S->waiting.end(), r));
+        r->state = CANCELLED;
+        r->status = LATER_ERR_CANCELLED;
+        r->t_done = now_s();
+        progress(*S);
+        return LATER_OK;
+    }
+    if (r->state == RUNNING_QR || r->state == RUNNING_QR2 || r->state == RUNNING_SOLVE) {
+        r->cancel = true;
+        progress(*S);
+        return LATER_OK;
+    }
+    return LATER_ERR_SHAPE;
+}
+
 LATER_API int later_server_release(void *srv, uint64_t id)
 {
     Server *S = (Server *)srv;
@@ -524,7 +558,7 @@ LATER_API int later_server_release(void *srv, uint64_t id)
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return LATER_ERR_SHAPE;
     Req *r = it->second;
-    if (r->state != DONE && r->state != FAILED) return LATER_ERR_SHAPE;
+    if (r->state != DONE && r->state != FAILED && r->state != CANCELLED) return LATER_ERR_SHAPE;
     if (r->off >= 0) pool_free(*S, r->off, r->bytes);
     if (r->ev) hipEventDestroy(r->ev);
     S->reqs.erase(it);

@@ -4,6 +4,8 @@ Oracle: FP64 LAPACK least squares (numpy lstsq) on the same inputs.  The
 server's results must be bitwise identical whether a request runs alone or
 concurrently with others (its kernels depend only on its own shape).
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -141,3 +143,44 @@ def test_server_budget_queues_and_rejects(later):
         srv.release(rid)
     finally:
         srv.close()
+
+
+def test_server_cancel(later):
+    """Cancelling a waiting request removes it from the queue at once; a
+    running one finishes its enqueued stage and is retired at the next poll;
+    a finished one cannot be cancelled.  The others' results are those they
+    give alone, and the pool drains to zero."""
+    import torch
+    probs = [_problem(1500, 200, 3, 100.0, 40 + i) for i in range(4)]
+    srv = later.Server(slots=1, hbm_budget_bytes=2 << 30)
+    try:
+        devs = [(_dev(later, A), _dev(later, B)) for A, B, _ in probs]
+        ids = [srv.submit(A, B) for A, B in devs]
+        # one slot: request 0 runs, 1..3 wait
+        assert srv.result(ids[0])["state"] in (1, 2, 5)
+        assert srv.result(ids[2])["state"] == 0
+        srv.cancel(ids[2])
+        assert srv.result(ids[2])["state"] == 6 and srv.result(ids[2])["status"] == later.ERR_CANCELLED
+        srv.cancel(ids[0])  # running: retired after its current stage
+        t0 = time.time()
+        while any(srv.result(i)["state"] not in (3, 4, 6) for i in ids):
+            srv.poll()
+            assert time.time() - t0 < 60, "server did not drain"
+        torch.cuda.synchronize()
+        assert srv.result(ids[0])["state"] == 6
+        assert [srv.result(i)["state"] for i in (ids[1], ids[3])] == [3, 3]
+        with pytest.raises(later.LaterError):
+            srv.cancel(ids[1])
+        got = {}
+        for j in (1, 3):
+            A, B, _ = probs[j]
+            X, R = srv.fetch(ids[j], A.shape[0], A.shape[1], B.shape[1])
+            got[j] = (_np(X), _np(R))
+        for i in ids:
+            srv.release(i)
+        assert srv.stats()["used_bytes"] == 0
+    finally:
+        srv.close()
+    for j in (1, 3):
+        alone, _ = _run_server(later, [probs[j]], slots=1, budget=2 << 30)
+        assert np.array_equal(alone[0][0], got[j][0]) and np.array_equal(alone[0][1], got[j][1])
