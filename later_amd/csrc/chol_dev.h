@@ -38,6 +38,7 @@ struct CholWork {
     double *p2;  // kMaxGroups x P: partial Grams of Q1
     double *r1;  // W x W, row-major: R1 (upper)
     double *g;   // P: a reduced Gram (more than kDirect partials)
+    double *ri;  // W x W, row-major: the inverse factor the apply kernel multiplies by
 };
 
 template <int W> __host__ __device__ inline CholWork carve(float *ws)
@@ -48,12 +49,13 @@ template <int W> __host__ __device__ inline CholWork carve(float *ws)
     c.p2 = c.p1 + (int64_t)kMaxGroups * P;
     c.r1 = c.p2 + (int64_t)kMaxGroups * P;
     c.g = c.r1 + W * W;
+    c.ri = c.g + P;
     return c;
 }
 
 template <int W> constexpr int64_t work_floats()
 {
-    return ((int64_t)2 * kMaxGroups * Shape<W>::P + W * W + Shape<W>::P) * 2 + 64;
+    return ((int64_t)2 * kMaxGroups * Shape<W>::P + 2 * W * W + Shape<W>::P) * 2 + 64;
 }
 
 // Packed index e of the upper triangle (column-major, i <= j) -> (i, j).
