@@ -112,9 +112,9 @@ typedef struct later_opts {
 /* Defaults reproduce the reference form: FP16 GEMMs above 128 columns, FP32
  * below, FP32 block-MGS TSQR panel, no re-orthogonalisation. */
 void later_opts_default(later_opts *o);
-/* The accuracy-recovering mixed-precision form: FP16X3 first pass (FP64
- * folds every 256 rows), FP16X6 second-pass projections (folds every 32
- * rows), Cholesky QR leaves of up to 64 columns (one pass per leaf,
+/* The accuracy-recovering mixed-precision form: FP16X3 first pass (FP32
+ * accumulation, chunk 0), FP16X6 second-pass projections (FP64 folds every
+ * 32 rows), Cholesky QR leaves of up to 64 columns (one pass per leaf,
  * LATER_PANEL_CHOLQR1), second full pass. */
 void later_opts_mixed(later_opts *o);
 /* The plain-FP32 form: FP32 MFMA everywhere, reference panel, one pass. */
@@ -169,6 +169,20 @@ int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, i
 #define LATER_NOALLOC_SCRATCH_FLOATS (40 * 1024 * 1024)
 int later_rgsqrf_noalloc(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, int ldr,
                          const later_opts *opts, later_info *info);
+
+/* Out-of-core QR, replacing later_oc_qr_rec (include/LATER.h:49,
+ * QR/later_oc_qr.cu:29-94): A (m x n, lda) and R (n x n, ldr) are HOST
+ * memory (hipHostMalloc'd memory lets copies overlap products).  The
+ * reference's Mem_pool argument becomes `device_bytes`, the device memory
+ * the call may use (allocated once per call).  The recursion splits columns
+ * as the in-core one does until a column block fits, factors it in core with
+ * `opts`, and runs each wider node as a block Gram-Schmidt step over row
+ * blocks streamed through the device (block CGS2 when opts->reorth != 0).
+ * Synchronous: returns when A holds Q and R holds R.  LATER_ERR_WORKSPACE
+ * (before anything is copied) when a 32-column leaf or a 256-row node block
+ * does not fit the budget. */
+int later_oc_qr_rec(laterCtxt ctxt, int m, int n, float *A, int lda, float *R, int ldr,
+                    int64_t device_bytes, const later_opts *opts, later_info *info);
 
 /* Graph-captured plans (no counterpart in the reference, which launches
  * eagerly): later_plan_create records every kernel of one later_rgsqrf_ex

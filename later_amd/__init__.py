@@ -72,6 +72,8 @@ _SIGNATURES = {
                              ctypes.POINTER(later_opts), ctypes.POINTER(later_info)]),
     "later_rgsqrf_noalloc": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, ctypes.POINTER(later_opts),
                                   ctypes.POINTER(later_info)]),
+    "later_oc_qr_rec": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _i64, ctypes.POINTER(later_opts),
+                             ctypes.POINTER(later_info)]),
     "later_read_status": (_i, [laterCtxt, _vp, ctypes.POINTER(_i)]),
     "later_caqr_panel": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _vp, _i64, _i]),
     "later_gemm_workspace": (_i, [_i, _i, _i, _i, _i, _i, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
@@ -248,6 +250,20 @@ def rgsqrf(A, R=None, o: later_opts | None = None, work=None, stream=None, info:
                                ctypes.byref(o) if o is not None else None, ctypes.byref(inf))
     _check(rc, "later_rgsqrf_ex")
     return (R, inf) if info else R
+
+
+def oc_qr_rec(A, R, device_bytes: int, o: later_opts | None = None, stream=None, info: bool = False):
+    """Out-of-core QR of the HOST column-major FP32 tensor A in place (A <- Q,
+    R <- R, both CPU tensors; pinned memory lets copies overlap products)
+    using at most device_bytes of device memory.  Synchronous."""
+    if A.device.type != "cpu" or R.device.type != "cpu":
+        raise ValueError("oc_qr_rec takes host tensors")
+    m, n = A.shape
+    inf = later_info()
+    rc = lib().later_oc_qr_rec(ctxt(stream), m, n, A.data_ptr(), _ld(A), R.data_ptr(), _ld(R), int(device_bytes),
+                               ctypes.byref(o) if o is not None else None, ctypes.byref(inf))
+    _check(rc, "later_oc_qr_rec")
+    return inf if info else None
 
 
 class Plan:

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblater_amd.so")
-SOURCES = ["gen.hip", "panel.hip", "gemm.hip", "check.hip", "rgsqrf.hip", "prof.hip", "ls.hip", "serve.hip", "plan.hip", "batch.hip", "panel_chol.hip", "noalloc.hip", "batch_wg.hip"]
+SOURCES = ["gen.hip", "panel.hip", "gemm.hip", "check.hip", "rgsqrf.hip", "prof.hip", "ls.hip", "serve.hip", "plan.hip", "batch.hip", "panel_chol.hip", "noalloc.hip", "batch_wg.hip", "ooc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-fvisibility=hidden",

@@ -114,10 +114,6 @@ __device__ __forceinline__ void mgs_block_body(int m, int w, float *A, int64_t l
     }
 }
 
-// A_b <- A_b * W_b for every row block b, W_b = rows b*w .. b*w+w-1 of the
-// stacked factor (ld ldw).  Each thread owns whole rows, so the in-place
-// update never reads a value another thread writes (the reference relies on
-// an aliased batched Sgemm here, QR/panel.cu:97-102).
 template <typename Acc>
 __global__ __launch_bounds__(kThreads) void mgs_block_kernel(int m, int w, float *A, int lda, float *Rout,
                                                              int ldr, int bh, int nblk, int *status)
@@ -133,6 +129,10 @@ __global__ __launch_bounds__(kThreads) void mgs_block_grouped_kernel(const Panel
     mgs_block_body<Acc>(a.m, a.w, a.A, a.lda, a.W, a.ldw, kPanelBlockRows, a.nblk, a.status, t.y);
 }
 
+// A_b <- A_b * W_b for every row block b, W_b = rows b*w .. b*w+w-1 of the
+// stacked factor (ld ldw).  Each thread owns whole rows, so the in-place
+// update never reads a value another thread writes (the reference relies on
+// an aliased batched Sgemm here, QR/panel.cu:97-102).
 template <typename Acc>
 __device__ __forceinline__ void tsqr_apply_body(int m, int w, float *A, int64_t lda, const float *W, int64_t ldw,
                                                 int bh, int nblk, const int b)
@@ -167,18 +167,49 @@ __device__ __forceinline__ void tsqr_apply_body(int m, int w, float *A, int64_t 
 }
 
 template <typename Acc>
-__global__ __launch_bounds__(256) void tsqr_apply_kernel(int m, int w, float *A, int lda, const float *W, int ldw,
-                                                         int bh, int nblk)
-{
-    tsqr_apply_body<Acc>(m, w, A, lda, W, ldw, bh, nblk, blockIdx.x);
-}
-
-template <typename Acc>
 __global__ __launch_bounds__(256) void tsqr_apply_grouped_kernel(const PanelArgs *args, const int2 *tasks)
 {
     const int2 t = tasks[blockIdx.x];
     const PanelArgs a = args[t.x];
     tsqr_apply_body<Acc>(a.m, a.w, a.A, a.lda, a.W, a.ldw, kPanelBlockRows, a.nblk, t.y);
+}
+
+// The same product over 64-row tiles of the panel (four waves per tile: the
+// tile is staged in LDS, wave q forms output columns q, q+4, ...), so that a
+// 16384-row panel spreads over 256 workgroups instead of 64.  Every output
+// keeps the row-per-thread form's order of summation over k, so the two give
+// the same bits.  A tile never straddles two 256-row blocks; the tiles of a
+// merged tail belong to the last block.
+constexpr int kApplyRows = 64;
+
+template <typename Acc>
+__global__ __launch_bounds__(256) void tsqr_apply_tile_kernel(int m, int w, float *A, int lda, const float *W,
+                                                              int ldw, int bh, int nblk)
+{
+    __shared__ float Ws[kCols][kCols];
+    __shared__ float Vs[kCols][kApplyRows];
+    const int row0 = blockIdx.x * kApplyRows;
+    const int b = min(row0 / bh, nblk - 1);
+    const int rows = min(kApplyRows, m - row0);
+    const float *Wb = W + (int64_t)b * w;
+    for (int i = threadIdx.x; i < w * w; i += 256) {
+        const int x = i % w, y = i / w;
+        Ws[x][y] = Wb[x + (int64_t)y * ldw];
+    }
+    const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+    float *At = A + row0;
+    for (int c = q; c < w; c += 4) Vs[c][r] = (r < rows) ? At[r + (int64_t)c * lda] : 0.0f;
+    __syncthreads();
+    float v[kCols];
+#pragma unroll
+    for (int k = 0; k < kCols; ++k) v[k] = (k < w) ? Vs[k][r] : 0.0f;
+    for (int j = q; j < w; j += 4) {
+        Acc s = 0;
+#pragma unroll
+        for (int k = 0; k < kCols; ++k)
+            if (k < w) s += (Acc)v[k] * (Acc)Ws[k][j];
+        if (r < rows) At[r + (int64_t)j * lda] = (float)s;
+    }
 }
 
 struct Blocks {
@@ -276,11 +307,12 @@ int panel_tsqr(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr
     HIP_OK(hipGetLastError());
     int rc = panel_tsqr(s, ldst, w, stack, ldst, R, ldr, work + need, lwork - need, fp64, status, launches);
     if (rc) return rc;
+    const int tiles = (m + kApplyRows - 1) / kApplyRows;
     if (fp64)
-        hipLaunchKernelGGL(tsqr_apply_kernel<double>, dim3(p.nblk), dim3(256), 0, s, m, w, A, lda, stack, ldst,
+        hipLaunchKernelGGL(tsqr_apply_tile_kernel<double>, dim3(tiles), dim3(256), 0, s, m, w, A, lda, stack, ldst,
                            bh, p.nblk);
     else
-        hipLaunchKernelGGL(tsqr_apply_kernel<float>, dim3(p.nblk), dim3(256), 0, s, m, w, A, lda, stack, ldst,
+        hipLaunchKernelGGL(tsqr_apply_tile_kernel<float>, dim3(tiles), dim3(256), 0, s, m, w, A, lda, stack, ldst,
                            bh, p.nblk);
     if (launches) ++*launches;
     HIP_OK(hipGetLastError());

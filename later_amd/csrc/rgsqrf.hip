@@ -314,14 +314,18 @@ LATER_API void later_opts_fp32(later_opts *o)
     o->gemm_pass2 = LATER_GEMM_F32;
 }
 
-// Mixed form: FP16 hi/lo products with FP64 folds every 256 rows in the
-// first pass; three-piece products folded every 32 rows in the second pass,
-// whose projections set the orthogonality floor; Cholesky QR leaves of up to
-// 64 columns, one pass each (the second factorisation repairs what a second
-// Cholesky QR pass of a first-pass leaf would: at 16384^2 the orthogonality
-// is 8.77e-6 either way).  At 16384^2 (N(0,1)) this gives
-// ||I - Q^T Q||_F = 8.8e-6 and ||A - QR||_F / ||A||_F = 3.4e-7; the same
-// with two-piece second-pass products gives 1.3e-5.
+// Mixed form: FP16 hi/lo products accumulated in FP32 over the whole depth
+// in the first pass (its projections only have to leave a Q that the second
+// pass can repair: FP64 folds every 256 rows change the final orthogonality
+// from 8.66e-6 to 8.60e-6 at 16384^2 and cost 2.5 ms); three-piece products
+// folded every 32 rows in the second pass, whose projections set the
+// orthogonality floor; Cholesky QR leaves of up to 64 columns, one pass each
+// (the second factorisation repairs what a second Cholesky QR pass of a
+// first-pass leaf would: at 16384^2 the orthogonality is 8.77e-6 either way).
+// At 16384^2 (N(0,1)) this gives ||I - Q^T Q||_F = 8.7e-6 and
+// ||A - QR||_F / ||A||_F = 3.4e-7 (graded singular values 1 .. 1e-4: 9.6e-6
+// and 5.8e-7); single FP16 first-pass projections give 1.02e-5 (graded
+// 4.1e-5) and two-piece second-pass products 1.3e-5.
 LATER_API void later_opts_mixed(later_opts *o)
 {
     later_opts_default(o);
@@ -329,7 +333,7 @@ LATER_API void later_opts_mixed(later_opts *o)
     o->gemm_pass2 = LATER_GEMM_F16X6;
     o->panel = LATER_PANEL_CHOLQR1;
     o->reorth = LATER_REORTH_PASS;
-    o->chunk = 256;
+    o->chunk = 0;
     o->chunk2 = 32;
 }
 

@@ -404,6 +404,7 @@ LATER_API int later_server_create(int slots, int64_t hbm_budget_bytes, const lat
     }
     later_opts_mixed(&S->qr_robust);
     S->qr_robust.reorth = LATER_REORTH_NODE | LATER_REORTH_PASS;
+    S->qr_robust.chunk = 256;  // ill-conditioned requests: FP64 folds in both passes
     S->pool_bytes = hbm_budget_bytes / 256 * 256;
     if (hipMalloc((void **)&S->pool, S->pool_bytes) != hipSuccess) { delete S; return LATER_ERR_DEVICE; }
     if (hipHostMalloc((void **)&S->cond_pinned, 2 * sizeof(double) * slots) != hipSuccess) {

@@ -33,7 +33,8 @@ def test_options(later):
     o = later.opts("fp32")
     assert o.gemm == later.GEMM_F32
     o = later.opts("mixed")
-    assert o.gemm == later.GEMM_F16X3 and o.reorth == later.REORTH_PASS and o.chunk > 0
+    assert o.gemm == later.GEMM_F16X3 and o.reorth == later.REORTH_PASS
+    assert o.gemm_pass2 == later.GEMM_F16X6 and o.chunk == 0 and o.chunk2 == 32
 
 
 @pytest.mark.parametrize("m,n", [(16384, 16384), (8192, 1024), (1048576, 128), (262144, 256), (100, 7)])
@@ -73,3 +74,28 @@ def test_ex_rejects_without_touching_device(later):
     need = later.workspace_size(100, 20)
     assert L.later_rgsqrf_ex(c, 100, 20, 1 << 24, 100, 2 << 20, 20, (2 << 20) + 64, need, None, 0, None,
                              None) == later.ERR_ALIAS
+
+
+@pytest.mark.parametrize("form", ["fp32", "reference", "mixed"])
+def test_oc_budget_checked_before_any_copy(later, form):
+    """The out-of-core driver's dry run rejects a budget below what a 32-column
+    leaf or a 256-row node block needs (LATER_ERR_WORKSPACE), before touching
+    the device or the host matrices; bad shapes and options come first."""
+    import numpy as np
+    L = later.lib()
+    c = later.laterCtxt(None, None)
+    m, n = 65536, 256
+    A = np.full((n, m), 3.0, np.float32)
+    R = np.full((n, n), 7.0, np.float32)
+    o = later.opts(form)
+    info = later.later_info()
+    rc = L.later_oc_qr_rec(c, m, n, A.ctypes.data, m, R.ctypes.data, n, 4 << 20, ctypes.byref(o),
+                           ctypes.byref(info))
+    assert rc == later.ERR_WORKSPACE and info.status == rc
+    assert np.all(A == 3.0) and np.all(R == 7.0)
+    assert L.later_oc_qr_rec(c, 10, 20, A.ctypes.data, 10, R.ctypes.data, 20, 1 << 30, None, None) \
+        == later.ERR_SHAPE
+    for kv in ({"chunk": 17}, {"gemm": 9}, {"gemm_update": 4}, {"panel": 5}):
+        bad = later.opts(form, **kv)
+        assert L.later_oc_qr_rec(c, m, n, A.ctypes.data, m, R.ctypes.data, n, 1 << 30, ctypes.byref(bad),
+                                 None) == later.ERR_OPTION
