@@ -528,19 +528,21 @@ def main():
             f"({time.time() - t0:.1f} s)")
 
     mixed, fp32 = res["mixed"], res["fp32"]
-    # Dominant kernel of the mixed form: the FP16 hi/lo node update (and the
-    # R2 R1 composition) on pre-split operands, gemm_lds_kernel<3, false, 256,
-    # 4>, timed per launch with HIP events on its stream.  achieved = its
-    # algorithmic flops (2 m n1 n2 per update, n^3 / 3 for the triangular
-    # composition) summed, over its summed duration; each such flop costs
-    # three MFMA flops (hi*hi, hi*lo, lo*hi).
+    # Dominant kernel of the mixed form: the streaming FP16 hi/lo kernel on
+    # pre-split operands and 256-row tiles, gemm_lds_kernel<3, false, 256, 4>
+    # (the node updates, the R2 R1 composition and the first pass's large
+    # projections), timed per launch with HIP events on its stream.  achieved
+    # = its algorithmic flops (2 m n1 n2 per update or projection, n^3 / 3 for
+    # the triangular composition) summed, over its summed duration; each such
+    # flop costs three MFMA flops (hi*hi, hi*lo, lo*hi).
     pr = mixed["prof"]
     k = pr["gemm_update_x3"]
     achieved = k["flops"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
     traffic = read_traffic(k)
     roofline = {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK_TFLOPS["f16x3"], "unit": "TFLOP/s",
                 "frac": achieved / MFMA_PEAK_TFLOPS["f16x3"], "traffic": traffic,
-                "kernel": "gemm_lds_kernel<3,false,256,4> (node update A2 -= Q1 R12 and R2 R1, FP16 hi/lo pieces)",
+                "kernel": "gemm_lds_kernel<3,false,256,4> (node updates A2 -= Q1 R12, R2 R1, first-pass Q1^T A2; "
+                          "FP16 hi/lo pieces)",
                 "launches": k["launches"], "kernel_ms": k["ms"], "avg_launch_us": 1e3 * k["ms"] / max(k["launches"], 1),
                 "algorithmic_bytes_per_launch": k["bytes"] / max(k["launches"], 1),
                 "mfma_flops_issued_tflops": 3.0 * achieved}

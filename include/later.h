@@ -253,7 +253,9 @@ int later_read_status(laterCtxt ctxt, const float *work, int *device_status);
 /* Panel of the reference alone: mgs_caqr_panel_256x32 (QR/panel.cu:65-134)
  * generalised to w <= 32 columns.  A (m x w) becomes Q, R (w x w).
  * fp64_sums: 0 the reference's FP32 block MGS, 1 block MGS with FP64 sums,
- * 2 Cholesky QR twice with FP64 Gram factors (LATER_PANEL_CHOLQR2). */
+ * 2 Cholesky QR twice with FP64 Gram factors (LATER_PANEL_CHOLQR2).
+ * later_caqr_panel_workspace: the FP32 `work` elements it needs. */
+int later_caqr_panel_workspace(int m, int w, int fp64_sums, int64_t *lwork);
 int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, float *R, int ldr,
                      float *work, int64_t lwork, int fp64_sums);
 

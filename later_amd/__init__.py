@@ -75,6 +75,7 @@ _SIGNATURES = {
     "later_oc_qr_rec": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _i64, ctypes.POINTER(later_opts),
                              ctypes.POINTER(later_info)]),
     "later_read_status": (_i, [laterCtxt, _vp, ctypes.POINTER(_i)]),
+    "later_caqr_panel_workspace": (_i, [_i, _i, _i, ctypes.POINTER(_i64)]),
     "later_caqr_panel": (_i, [laterCtxt, _i, _i, _vp, _i, _vp, _i, _vp, _i64, _i]),
     "later_gemm_workspace": (_i, [_i, _i, _i, _i, _i, _i, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "later_gemm_proj": (_i, [laterCtxt, _i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _vp, _i64, _vp, _i64]),
@@ -414,6 +415,13 @@ def read_status(work, stream=None) -> int:
     return int(st.value)
 
 
+def caqr_panel_workspace(m: int, w: int, fp64=False) -> int:
+    """FP32 elements of `work` that caqr_panel needs."""
+    lw = _i64()
+    _check(lib().later_caqr_panel_workspace(m, w, int(fp64), ctypes.byref(lw)), "later_caqr_panel_workspace")
+    return int(lw.value)
+
+
 def caqr_panel(A, R=None, fp64=False, work=None, stream=None):
     """mgs_caqr_panel_256x32 (QR/panel.cu:65-134) for an m x w panel, w <= 32.
     fp64: False/0 FP32 block MGS, True/1 FP64 sums, 2 Cholesky QR twice."""
@@ -422,7 +430,7 @@ def caqr_panel(A, R=None, fp64=False, work=None, stream=None):
     if R is None:
         R = colmajor(w, w, device=A.device)
     if work is None:
-        work = torch.empty(64 + max(600000, 40 * m), device=A.device, dtype=torch.float32)
+        work = torch.empty(caqr_panel_workspace(m, w, fp64), device=A.device, dtype=torch.float32)
     _check(lib().later_caqr_panel(ctxt(stream), m, w, A.data_ptr(), _ld(A), R.data_ptr(), _ld(R),
                                   work.data_ptr(), work.numel(), int(fp64)), "later_caqr_panel")
     return R

@@ -13,7 +13,7 @@ constexpr int kWMax = 64;                 // max panel width
 constexpr int kThreads = 256;             // threads per workgroup (four waves)
 constexpr int kTT = 1;                    // 16-row tiles per wave in a block
 constexpr int kRows = 64 * kTT;           // rows per block (workgroup b takes blocks b, b + G, ...)
-constexpr int kMaxGroups = 256;           // workgroups (and Gram partials) per launch
+constexpr int kMaxGroups = 512;           // Gram workgroups (and partials) per launch: two per CU
 constexpr int kDirect = 32;               // up to this many partials, every workgroup adds them itself
 constexpr double kPivotRel = 64.0 * 2.220446049250313e-16;
 

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kThreads) void chol_apply_kernel(int m, int w, floa
 }
 
 // One Cholesky QR pass over the rows of this workgroup, the factor formed
-// by every workgroup (panels of at most kMaxGroups 64-row blocks: one
+// by every workgroup (panels of at most kFusedBlocks 64-row blocks: one
 // launch less than the separate factor kernel).
 //   PASS 1: Gram of A -> R1 (stored to r1 by workgroup 0); Q1 = A R1^{-1}
 //           stored; partials p2 of the stored Q1.
@@ -426,6 +426,7 @@ __global__ __launch_bounds__(kThreads) void chol_apply_fused_kernel(int m, int w
 // Workgroups of the apply kernels without a partial Gram: several per CU, so
 // tall panels keep enough loads in flight.
 constexpr int kMaxApplyGroups = 1024;
+constexpr int kFusedBlocks = 256;
 
 template <int W>
 int launch_cholqr(hipStream_t s, int m, int w, float *A, int lda, float *R, int ldr, float *work, int *status,
@@ -434,10 +435,10 @@ int launch_cholqr(hipStream_t s, int m, int w, float *A, int lda, float *R, int 
     const CholWork cw = carve<W>(work);
     const int nblk = (m + kRows - 1) / kRows;
     const int Ga = nblk < kMaxApplyGroups ? nblk : kMaxApplyGroups;
-    // Up to kMaxGroups blocks every apply workgroup forms the factor itself
+    // Up to kFusedBlocks blocks every apply workgroup forms the factor itself
     // (the latency of a separate factor launch outweighs the redundancy);
     // taller panels factor once and apply with more workgroups.
-    const bool fused = nblk <= kMaxGroups;
+    const bool fused = nblk <= kFusedBlocks;
     const dim3 rgrid((Shape<W>::P + 31) / 32);
     int n = 0;
     hipLaunchKernelGGL(chol_gram_kernel<W>, dim3(G), dim3(kThreads), 0, s, m, w, (const float *)A, (int64_t)lda, work);

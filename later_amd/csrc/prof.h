@@ -4,9 +4,10 @@
 #pragma once
 #include "common.h"
 
-// PROF_UPDATE_X3: the node updates of the FP16 hi/lo form on pre-split
-// operands with 128 x 128 tiles (one kernel, the mixed form's largest);
-// PROF_GEMM_UPDATE holds every other update.
+// PROF_UPDATE_X3: every launch of the streaming FP16 hi/lo kernel on 256-row
+// tiles (pre-split operands, no FP64 folds; the mixed form's largest): the
+// node updates, the R2 R1 composition and the first pass's projections of
+// enough tiles; PROF_GEMM_UPDATE and PROF_GEMM_PROJ hold the other products.
 enum { PROF_GEMM_PROJ = 0, PROF_GEMM_UPDATE = 1, PROF_PANEL = 2, PROF_OTHER = 3, PROF_UPDATE_X3 = 4,
        PROF_NCLASS = 5 };
 

@@ -502,12 +502,21 @@ LATER_API int later_read_status(laterCtxt ctxt, const float *work, int *device_s
     return LATER_OK;
 }
 
+LATER_API int later_caqr_panel_workspace(int m, int w, int fp64_sums, int64_t *lwork)
+{
+    if (w < 1 || w > 32 || m < w) return LATER_ERR_SHAPE;
+    if (fp64_sums < 0 || fp64_sums > 2) return LATER_ERR_OPTION;
+    if (lwork) *lwork = kHeaderFloats + (fp64_sums == 2 ? panel_chol_work_floats(w) : panel_work_floats(m, w));
+    return LATER_OK;
+}
+
 LATER_API int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, float *R, int ldr, float *work,
                                int64_t lwork, int fp64_sums)
 {
     if (w < 1 || w > 32 || m < w || lda < m || ldr < w || !A || !R) return LATER_ERR_SHAPE;
-    if (fp64_sums < 0 || fp64_sums > 2) return LATER_ERR_OPTION;
-    const int64_t need = kHeaderFloats + (fp64_sums == 2 ? panel_chol_work_floats(w) : panel_work_floats(m, w));
+    int64_t need = 0;
+    const int rc = later_caqr_panel_workspace(m, w, fp64_sums, &need);
+    if (rc) return rc;
     if (!work || lwork < need) return LATER_ERR_WORKSPACE;
     hipStream_t s = ctx_stream(ctxt);
     hipLaunchKernelGGL(status_clear_kernel, dim3(1), dim3(64), 0, s, (int *)work);

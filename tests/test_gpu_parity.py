@@ -450,6 +450,6 @@ def test_cholqr2_panel_breakdown(later, oracle):
     A = oracle.gen_uniform(2000, 16, 0, 0)
     A[:, 4] = A[:, 3]  # rank deficient
     Ad = _dev(later, A)
-    work = torch.empty(64 + 600000, device="cuda")
+    work = torch.empty(later.caqr_panel_workspace(2000, 16, 2), device="cuda")
     later.caqr_panel(Ad, fp64=2, work=work)
     assert later.read_status(work) & 1
