@@ -107,14 +107,18 @@ typedef struct later_opts {
                          nmin (F32 .. F16X3), or -1: paired with the
                          projection precision (F16X6 and FP8 -> F16X3, the
                          others the same) */
+    int nmin2;        /* the second pass's projections use FP32 products at
+                         nodes of width <= nmin2 (0: nmin) and its updates at
+                         width <= max(nmin, nmin2) */
 } later_opts;
 
 /* Defaults reproduce the reference form: FP16 GEMMs above 128 columns, FP32
  * below, FP32 block-MGS TSQR panel, no re-orthogonalisation. */
 void later_opts_default(later_opts *o);
 /* The accuracy-recovering mixed-precision form: FP16X3 first pass (FP32
- * accumulation, chunk 0), FP16X6 second-pass projections (FP64 folds every
- * 32 rows), Cholesky QR leaves of up to 64 columns (one pass per leaf,
+ * accumulation, chunk 0) above 256 columns, FP16X6 second-pass projections
+ * above 128 columns (FP64 folds every 32 rows), FP32 products below,
+ * Cholesky QR leaves of up to 64 columns (one pass per leaf,
  * LATER_PANEL_CHOLQR1), second full pass. */
 void later_opts_mixed(later_opts *o);
 /* The plain-FP32 form: FP32 MFMA everywhere, reference panel, one pass. */

@@ -45,12 +45,12 @@ class laterCtxt(ctypes.Structure):
 class later_opts(ctypes.Structure):
     _fields_ = [("gemm", ctypes.c_int), ("gemm_pass2", ctypes.c_int), ("panel", ctypes.c_int),
                 ("reorth", ctypes.c_int), ("nmin", ctypes.c_int), ("chunk", ctypes.c_int),
-                ("chunk2", ctypes.c_int), ("gemm_update", ctypes.c_int)]
+                ("chunk2", ctypes.c_int), ("gemm_update", ctypes.c_int), ("nmin2", ctypes.c_int)]
 
     def __repr__(self):
         return ("later_opts(gemm=%d, gemm_pass2=%d, panel=%d, reorth=%d, nmin=%d, chunk=%d, chunk2=%d, "
-                "gemm_update=%d)" % (self.gemm, self.gemm_pass2, self.panel, self.reorth, self.nmin, self.chunk,
-                                     self.chunk2, self.gemm_update))
+                "gemm_update=%d, nmin2=%d)" % (self.gemm, self.gemm_pass2, self.panel, self.reorth, self.nmin,
+                                               self.chunk, self.chunk2, self.gemm_update, self.nmin2))
 
 
 class later_info(ctypes.Structure):

@@ -190,10 +190,10 @@ struct Builder {
         }
     }
 
-    void node(const std::vector<Mat *> &ms, int c0, int n1, int w2, int prec, int chunk, bool pass2)
+    void node(const std::vector<Mat *> &ms, int c0, int n1, int w2, int prec, int pu, int chunk, bool pass2)
     {
-        const bool sc = scaled(prec), ch64 = chunk > 0;
-        const int aexp = sc ? kQExp : 0;
+        const bool sc = scaled(prec), ch64 = chunk > 0, scu = scaled(pu);
+        const int aexp = sc ? kQExp : 0, aexp_u = scu ? kQExp : 0;
         // Projection R12 = Q1^T A2.
         if (sc) {
             std::vector<ColexpArgs> ca;
@@ -223,7 +223,7 @@ struct Builder {
         }
         gemm(prec, true, ch64, pa);
         // Update A2 -= Q1 R12.
-        if (sc) {
+        if (scu) {
             std::vector<ColexpArgs> ca;
             ncols_.clear();
             for (Mat *M : ms) {
@@ -243,13 +243,13 @@ struct Builder {
             g.A = p.A; g.lda = M->lda;
             g.B = p.C; g.ldb = p.ldc;
             g.C = (float *)p.B; g.ldc = M->lda;
-            g.a_exp = aexp; g.b_exp = p.b_exp;
+            g.a_exp = aexp_u; g.b_exp = scu ? M->exps : nullptr;
             g.kper = (n1 + 31) / 32 * 32;
             g.chunk_rows = 0;
-            g.partial = nullptr; g.update = ch64 ? 2 : 1; g.alpha = -1.0f; g.tri = 0;
+            g.partial = nullptr; g.update = (ch64 && pu != LATER_GEMM_F32) ? 2 : 1; g.alpha = -1.0f; g.tri = 0;
             ua.push_back(g);
         }
-        gemm(update_prec(prec), false, false, ua);
+        gemm(pu, false, false, ua);
     }
 
     void tree(const std::vector<Mat *> &ms, int c0, int wn, int prec, int chunk, bool pass2)
@@ -259,11 +259,15 @@ struct Builder {
             return;
         }
         const int n1 = split_n(wn);
-        const bool inner = wn <= o.nmin;
+        // FP32-product widths per pass, as the single-matrix recursion runs them.
+        const int nmin_p = (pass2 && o.nmin2 > 0) ? o.nmin2 : o.nmin;
+        const int nmin_u = pass2 ? std::max(o.nmin, nmin_p) : o.nmin;
+        const bool inner = wn <= nmin_p;
         const int p = inner ? LATER_GEMM_F32 : prec;
+        const int pu = wn <= nmin_u ? LATER_GEMM_F32 : update_prec(prec);
         const int ch = inner ? 0 : chunk;
         tree(ms, c0, n1, prec, chunk, pass2);
-        node(ms, c0, n1, wn - n1, p, ch, pass2);
+        node(ms, c0, n1, wn - n1, p, pu, ch, pass2);
         tree(ms, c0 + n1, wn - n1, prec, chunk, pass2);
     }
 
@@ -363,7 +367,7 @@ LATER_API int later_batch_plan_create(int count, const int *m, const int *n, flo
     // the FP64 sums only after the last k).
     auto prec_ok = [](int p) { return p >= LATER_GEMM_F32 && p <= LATER_GEMM_F16X6; };
     if (!prec_ok(o.gemm) || !prec_ok(o.gemm_pass2) || o.gemm_update != -1 || o.panel < 0 || o.panel > 3 ||
-        o.reorth < 0 || o.reorth > 3 || o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0 || o.chunk2 < 0 ||
+        o.reorth < 0 || o.reorth > 3 || o.nmin < 0 || o.nmin2 < 0 || o.chunk < 0 || (o.chunk % 32) != 0 || o.chunk2 < 0 ||
         (o.chunk2 % 32) != 0)
         return LATER_ERR_OPTION;
     int64_t need = 0;

@@ -344,7 +344,7 @@ LATER_API int later_oc_qr_rec(laterCtxt ctxt, int m, int n, float *A, int lda, f
     if (m < 1 || n < 1 || m < n || lda < m || ldr < n || !A || !R) rc = LATER_ERR_SHAPE;
     else if (c.o.gemm < LATER_GEMM_F32 || c.o.gemm > LATER_GEMM_FP8 || c.o.gemm_pass2 < LATER_GEMM_F32 ||
              c.o.gemm_pass2 > LATER_GEMM_FP8 || c.o.gemm_update < -1 || c.o.gemm_update > LATER_GEMM_F16X3 ||
-             c.o.panel < 0 || c.o.panel > 3 || c.o.reorth < 0 || c.o.reorth > 3 || c.o.nmin < 0 ||
+             c.o.panel < 0 || c.o.panel > 3 || c.o.reorth < 0 || c.o.reorth > 3 || c.o.nmin < 0 || c.o.nmin2 < 0 ||
              c.o.chunk < 0 || c.o.chunk % 32 || c.o.chunk2 < 0 || c.o.chunk2 % 32)
         rc = LATER_ERR_OPTION;
     if (!rc) {

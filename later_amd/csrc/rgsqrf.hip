@@ -17,6 +17,8 @@
 //                      R := R2 R1 = R1 + (R2 - I) R1.
 #include <string.h>
 
+#include <algorithm>
+
 #include "common.h"
 #include "gemm.h"
 #include "panel.h"
@@ -36,6 +38,12 @@ int update_prec(int p) { return (p == LATER_GEMM_F16X6 || p == LATER_GEMM_FP8) ?
 // The update precision of the nodes whose projections run `p`.
 int update_of(const later_opts &o, int p) { return o.gemm_update >= 0 ? o.gemm_update : update_prec(p); }
 bool update_ok(int p) { return p == -1 || (p >= LATER_GEMM_F32 && p <= LATER_GEMM_F16X3); }
+// FP32-product widths of a pass: the first pass runs nmin for both products;
+// the second runs nmin2 (0: nmin) for its projections, which set the final
+// orthogonality, and the wider of the two for its updates (FP32 products are
+// more accurate than the hi/lo pair and faster on these memory-bound shapes).
+int pass_nmin_p(const later_opts &o, bool second) { return (second && o.nmin2 > 0) ? o.nmin2 : o.nmin; }
+int pass_nmin_u(const later_opts &o, bool second) { return second ? std::max(o.nmin, pass_nmin_p(o, true)) : o.nmin; }
 
 struct Layout {
     int64_t header, exps, panel, gemm, snode, r2, tcopy, total;
@@ -90,8 +98,8 @@ Layout layout(int m, int n, const later_opts &o)
     L.exps = ((int64_t)n + 63) / 64 * 64;
     L.panel = chol_panel(o.panel) ? panel_chol_work_floats(n) : panel_work_floats_any(m);
     int64_t g = 0;
-    walk_gemm(m, n, o.nmin, o.chunk, &g);
-    if (o.reorth & LATER_REORTH_PASS) walk_gemm(m, n, o.nmin, o.chunk2 > 0 ? o.chunk2 : o.chunk, &g);
+    walk_gemm(m, n, pass_nmin_p(o, false), o.chunk, &g);
+    if (o.reorth & LATER_REORTH_PASS) walk_gemm(m, n, pass_nmin_p(o, true), o.chunk2 > 0 ? o.chunk2 : o.chunk, &g);
     L.gemm = g;
     const int n1 = split_n(n);
     L.snode = ((o.reorth & LATER_REORTH_NODE) && n > 32) ? (int64_t)n1 * (n - n1) : 0;
@@ -100,7 +108,7 @@ Layout layout(int m, int n, const later_opts &o)
     L.total = L.header + L.exps + L.panel + L.gemm + L.snode + L.r2 + L.tcopy;
     // The top node is the largest.
     int64_t ph = (o.gemm != LATER_GEMM_F32 && n > o.nmin) ? node_pieces(m, n, o.gemm, update_of(o, o.gemm)) : 0;
-    if ((o.reorth & LATER_REORTH_PASS) && o.gemm_pass2 != LATER_GEMM_F32 && n > o.nmin) {
+    if ((o.reorth & LATER_REORTH_PASS) && o.gemm_pass2 != LATER_GEMM_F32 && n > pass_nmin_p(o, true)) {
         const int64_t p2 = node_pieces(m, n, o.gemm_pass2, update_of(o, o.gemm_pass2));
         if (p2 > ph) ph = p2;
     }
@@ -127,7 +135,10 @@ struct Run {
     bool node_reorth;  // block CGS2 at the nodes of this pass
     int chunk;         // FP64 fold chunk of this pass
     bool chol_once;    // one Cholesky QR pass per leaf (the input is a Q)
+    int nmin_p;        // projections of nodes this wide or narrower run FP32 products
+    int nmin_u;        // the same for the updates
 };
+
 
 // Column-wise helpers on a (ceil(rows / 256), min(cols, 65535)) grid: thread
 // row i of columns j = blockIdx.y, blockIdx.y + gridDim.y, ...
@@ -184,11 +195,11 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
                           r.o.panel == LATER_PANEL_MGS64, r.status, &r.launches);
     }
     const int n1 = split_n(n), n2 = n - n1;
-    const bool inner = (n <= r.o.nmin);
-    const int p = inner ? LATER_GEMM_F32 : prec;               // projection precision
-    const int pu = inner ? LATER_GEMM_F32 : update_of(r.o, prec);  // update precision
+    const bool inner = (n <= r.nmin_p), inner_u = (n <= r.nmin_u);
+    const int p = inner ? LATER_GEMM_F32 : prec;                     // projection precision
+    const int pu = inner_u ? LATER_GEMM_F32 : update_of(r.o, prec);  // update precision
     const int chunk = inner ? 0 : r.chunk;
-    const bool f64 = chunk > 0;
+    const bool f64 = r.chunk > 0 && !inner_u;  // split updates subtract in FP64 when the pass folds
     int rc = qr_rec(r, m, n1, A, lda, R, ldr, prec, depth + 1);
     if (rc) return rc;
     float *A2 = A + (int64_t)n1 * lda;
@@ -200,7 +211,7 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     // once per node instead of once per tile.  FP8 projections are always
     // staged (their block scales need a pass over each 128-row block).
     const bool fp8 = p == LATER_GEMM_FP8;
-    const bool pre = r.pcs && !inner && n >= kPieceMinN && gemm_pieces(pu) > 0;
+    const bool pre = r.pcs && !inner_u && n >= kPieceMinN && gemm_pieces(pu) > 0;
     const bool pre_proj = fp8 ? (r.pcs && !inner) : (pre && proj_pieces(p) > 0);
     if (fp8 && !pre_proj) return LATER_ERR_WORKSPACE;
     Pieces q1h{}, a2h{}, q1t{}, r12h{};
@@ -305,6 +316,7 @@ LATER_API void later_opts_default(later_opts *o)
     o->chunk = 0;
     o->chunk2 = 0;
     o->gemm_update = -1;
+    o->nmin2 = 0;
 }
 
 LATER_API void later_opts_fp32(later_opts *o)
@@ -318,14 +330,17 @@ LATER_API void later_opts_fp32(later_opts *o)
 // in the first pass (its projections only have to leave a Q that the second
 // pass can repair: FP64 folds every 256 rows change the final orthogonality
 // from 8.66e-6 to 8.60e-6 at 16384^2 and cost 2.5 ms); three-piece products
-// folded every 32 rows in the second pass, whose projections set the
-// orthogonality floor; Cholesky QR leaves of up to 64 columns, one pass each
+// folded every 32 rows for the second pass's projections, which set the
+// orthogonality floor; FP32 products at nodes of up to 256 columns (128 for
+// the second pass's projections), which are memory-bound: the FP32 MFMA
+// runs them faster than the split forms and rounds less (262144 x 256:
+// 2.18 vs 2.85 ms); Cholesky QR leaves of up to 64 columns, one pass each
 // (the second factorisation repairs what a second Cholesky QR pass of a
-// first-pass leaf would: at 16384^2 the orthogonality is 8.77e-6 either way).
-// At 16384^2 (N(0,1)) this gives ||I - Q^T Q||_F = 8.7e-6 and
-// ||A - QR||_F / ||A||_F = 3.4e-7 (graded singular values 1 .. 1e-4: 9.6e-6
-// and 5.8e-7); single FP16 first-pass projections give 1.02e-5 (graded
-// 4.1e-5) and two-piece second-pass products 1.3e-5.
+// first-pass leaf would).  At 16384^2 (N(0,1)) the form gives
+// ||I - Q^T Q||_F ~ 8.7e-6 and ||A - QR||_F / ||A||_F = 3.4e-7 (graded
+// singular values 1 .. 1e-4: 9.6e-6 and 5.8e-7); single FP16 first-pass
+// projections give 1.02e-5 (graded 4.1e-5), two-piece second-pass
+// projections 1.3e-5, single FP16 second-pass updates 2.2e-4.
 LATER_API void later_opts_mixed(later_opts *o)
 {
     later_opts_default(o);
@@ -333,6 +348,8 @@ LATER_API void later_opts_mixed(later_opts *o)
     o->gemm_pass2 = LATER_GEMM_F16X6;
     o->panel = LATER_PANEL_CHOLQR1;
     o->reorth = LATER_REORTH_PASS;
+    o->nmin = 256;
+    o->nmin2 = 128;
     o->chunk = 0;
     o->chunk2 = 32;
 }
@@ -358,7 +375,8 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     int rc = LATER_OK;
     if (m < 1 || n < 1 || m < n || lda < m || ldr < n || !A || !R) rc = LATER_ERR_SHAPE;
     else if (!prec_ok(o.gemm) || !prec_ok(o.gemm_pass2) || !update_ok(o.gemm_update) || o.panel < 0 ||
-             o.panel > 3 || o.reorth < 0 || o.reorth > 3 || o.nmin < 0 || o.chunk < 0 || (o.chunk % 32) != 0 ||
+             o.panel > 3 || o.reorth < 0 || o.reorth > 3 || o.nmin < 0 || o.nmin2 < 0 || o.chunk < 0 ||
+             (o.chunk % 32) != 0 ||
              o.chunk2 < 0 || (o.chunk2 % 32) != 0 ||
              ((o.reorth & LATER_REORTH_NODE) && (o.gemm == LATER_GEMM_FP8 || o.gemm_pass2 == LATER_GEMM_FP8)))
         rc = LATER_ERR_OPTION;
@@ -424,6 +442,8 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     r.node_reorth = (o.reorth & LATER_REORTH_NODE) != 0;
     r.chunk = o.chunk;
     r.chol_once = false;
+    r.nmin_p = pass_nmin_p(o, false);
+    r.nmin_u = pass_nmin_u(o, false);
 
     hipLaunchKernelGGL(status_clear_kernel, dim3(1), dim3(64), 0, r.s, r.status);
     hipLaunchKernelGGL(zero_lower_kernel, grid_cols(n, n), dim3(256), 0, r.s, n, R, (int64_t)ldr);
@@ -436,6 +456,8 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
         ++r.launches;
         r.node_reorth = false;  // Q has condition number ~1: one projection per node suffices
         r.chol_once = true;     // and one Cholesky QR pass per leaf
+        r.nmin_p = pass_nmin_p(o, true);
+        r.nmin_u = pass_nmin_u(o, true);
         r.chunk = o.chunk2 > 0 ? o.chunk2 : o.chunk;
         rc = qr_rec(r, m, n, A, lda, R2, n, o.gemm_pass2, 0);
         if (!rc) {

@@ -35,6 +35,8 @@ def test_options(later):
     o = later.opts("mixed")
     assert o.gemm == later.GEMM_F16X3 and o.reorth == later.REORTH_PASS
     assert o.gemm_pass2 == later.GEMM_F16X6 and o.chunk == 0 and o.chunk2 == 32
+    assert o.nmin == 256 and o.nmin2 == 128
+    assert later.opts("reference").nmin2 == 0
 
 
 @pytest.mark.parametrize("m,n", [(16384, 16384), (8192, 1024), (1048576, 128), (262144, 256), (100, 7)])
