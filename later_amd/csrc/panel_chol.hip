@@ -93,13 +93,14 @@ template <int W> struct BlockRegs {
             }
         }
     }
-    __device__ __forceinline__ void store(float *Qs) const
+    __device__ __forceinline__ void store_ld(float *Qs, int ld) const
     {
         const int t = threadIdx.x;
 #pragma unroll
         for (int j = 0; j < NV; ++j)
-            *(floatx4 *)&Qs[((t >> 4) + 16 * j) * (kRows + 4) + 4 * (t & 15)] = v[j];
+            *(floatx4 *)&Qs[((t >> 4) + 16 * j) * ld + 4 * (t & 15)] = v[j];
     }
+    __device__ __forceinline__ void store(float *Qs) const { store_ld(Qs, kRows + 4); }
 };
 
 // Partial Gram of the rows of this workgroup (64-row blocks b, b + G, ...)
@@ -203,26 +204,39 @@ __global__ __launch_bounds__(kThreads) void chol_factor_kernel(int w, float *ws,
 }
 
 // Q = A R^{-1} over the rows of this workgroup (64-row blocks b, b + G, ...),
-// with R^{-1} from ri.  The product runs on FP64 MFMA as D = R^{-T} A^T per
-// 16 x 16 tile: the A operand (slices of R^{-1}) stays in registers, the B
-// operand is the block's rows, and D's lanes walk 16 consecutive rows of one
-// column (64-byte stores).  PASS 1 also forms the partial Gram of the stored
-// Q1 rows into p2[blockIdx.x] (then the grid is the Gram kernel's).
+// with R^{-1} from ri.  Each block is loaded with coalesced 16-byte row
+// loads two blocks ahead (as in the Gram kernel) and staged column-major in
+// LDS with a row stride of kRows + 16 floats, so that a wave's MFMA operand
+// reads (16 rows x 4 columns) cover 64 distinct banks.  The product runs on
+// FP64 MFMA as D = R^{-T} A^T per 16 x 16 tile: the A operand (slices of
+// R^{-1}) stays in registers, the B operand is the block's rows, and the
+// four column tiles' accumulation chains are interleaved k-step by k-step
+// (up to four independent MFMAs in flight); D's lanes walk 16 consecutive
+// rows of one column (64-byte stores).  PASS 1 also forms the partial Gram
+// of the stored Q1 rows into p2[blockIdx.x] (then the grid is the Gram
+// kernel's).
+constexpr int kApplyLd = kRows + 16;
+
 template <int PASS, int W>
 __global__ __launch_bounds__(kThreads) void chol_apply_kernel(int m, int w, float *A, int64_t lda, float *ws)
 {
     constexpr int CT = Shape<W>::CT, KS = Shape<W>::KS;
-    constexpr int kQs = (PASS == 1) ? W * (kRows + 4) * 4 : 16;
-    constexpr int kCb = (PASS == 1) ? 4 * Shape<W>::NT * 4 * 64 * 8 : 16;
-    __shared__ __attribute__((aligned(16))) char smem[kQs > kCb ? kQs : kCb];
-    float *Qs = (float *)smem;
+    constexpr int kAs = W * kApplyLd * 4;                       // the block, for the product
+    constexpr int kQs = (PASS == 1) ? W * (kRows + 4) * 4 : 0;  // the stored Q1 rows, for their Gram
+    constexpr int kCb = (PASS == 1) ? 4 * Shape<W>::NT * 4 * 64 * 8 : 0;
+    constexpr int kBytes = (kAs + kQs) > kCb ? (kAs + kQs) : kCb;
+    __shared__ __attribute__((aligned(16))) char smem[kBytes];
+    float *As = (float *)smem;
+    float *Qs = (float *)(smem + kAs);
     const CholWork cw = carve<W>(ws);
     const int nblk = (m + kRows - 1) / kRows;
     const int G = gridDim.x;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 
-    float av[kTT][KS];
-    if (blockIdx.x < nblk) load_block_mfma<W>(A, lda, m, w, blockIdx.x * kRows, av);
+    BlockRegs<W> r[2];
+    int b = blockIdx.x;
+    if (b < nblk) r[0].load(A, lda, m, w, b * kRows);
+    if (b + G < nblk) r[1].load(A, lda, m, w, (b + G) * kRows);
     // A-operand slices of R^{-1}: rv[ct][s] = R^{-1}[4 s + l/16][16 ct + l%16]
     // (zero for k > column, so column tile ct needs k-steps 0 .. 4 ct + 3).
     double rv[CT][KS];
@@ -235,31 +249,38 @@ __global__ __launch_bounds__(kThreads) void chol_apply_kernel(int m, int w, floa
     Gram<W> g;
     if constexpr (PASS == 1) gram_zero<W>(g);
 #pragma unroll 1
-    for (int b = blockIdx.x; b < nblk; b += G) {
-        if constexpr (PASS == 1) __syncthreads();  // Qs of the previous block has been read
-        const int r0 = b * kRows;
-#pragma unroll
-        for (int tt = 0; tt < kTT; ++tt) {
-            const int lrow = (wv * kTT + tt) * 16 + (lane & 15);  // row within the block
-            const int row = r0 + lrow;
-#pragma unroll
-            for (int ct = 0; ct < CT; ++ct) {
-                doublex4 d = doublex4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int s2 = 0; s2 < 4 * (ct + 1); ++s2)
-                    d = __builtin_amdgcn_mfma_f64_16x16x4f64(rv[ct][s2], (double)av[tt][s2], d, 0, 0, 0);
-                // d[r] = Q[row][16 ct + l/16 + 4 r]
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int col = 16 * ct + (lane >> 4) + 4 * r;
-                    const bool ok = row < m && col < w;
-                    const float q = ok ? (float)d[r] : 0.0f;
-                    if (ok) A[row + (int64_t)col * lda] = q;
-                    if constexpr (PASS == 1) Qs[col * (kRows + 4) + lrow] = q;
-                }
-            }
+    for (int it = 0; b < nblk; b += G, ++it) {
+        __syncthreads();  // the previous block's operands have been read
+        if (it & 1) r[1].store_ld(As, kApplyLd); else r[0].store_ld(As, kApplyLd);
+        __syncthreads();
+        if (b + 2 * G < nblk) {
+            if (it & 1) r[1].load(A, lda, m, w, (b + 2 * G) * kRows);
+            else r[0].load(A, lda, m, w, (b + 2 * G) * kRows);
         }
-        if (b + G < nblk) load_block_mfma<W>(A, lda, m, w, (b + G) * kRows, av);
+        const int lrow = wv * 16 + (lane & 15);  // row within the block
+        const int row = b * kRows + lrow;
+        float av[KS];
+#pragma unroll
+        for (int s2 = 0; s2 < KS; ++s2) av[s2] = As[(4 * s2 + (lane >> 4)) * kApplyLd + lrow];
+        doublex4 d[CT];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) d[ct] = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s2 = 0; s2 < KS; ++s2)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+                if (s2 < 4 * (ct + 1)) d[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(rv[ct][s2], (double)av[s2], d[ct], 0, 0, 0);
+        // d[ct][r] = Q[row][16 ct + l/16 + 4 r]
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = 16 * ct + (lane >> 4) + 4 * q;
+                const bool ok = row < m && col < w;
+                const float v = ok ? (float)d[ct][q] : 0.0f;
+                if (ok) A[row + (int64_t)col * lda] = v;
+                if constexpr (PASS == 1) Qs[col * (kRows + 4) + lrow] = v;
+            }
         if constexpr (PASS == 1) {
             // Gram of the stored (FP32) Q1 rows of this block.
             __syncthreads();
