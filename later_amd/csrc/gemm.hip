@@ -315,10 +315,25 @@ __device__ __forceinline__ void store_tile(const GemmArgs &g, int split, int row
 {
     const int lane = threadIdx.x & 63;
     const float oscale_a = ldexpf(1.0f, -g.a_exp);
+    const bool rmw = !g.partial && g.update;
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
         const int row = row0 + 32 * i + (lane & 31);
         if (row >= g.M) continue;
+        // An update reads all of its C values first, so that the loads are in
+        // flight together (interleaved with the stores they would wait one
+        // memory latency each: the compiler cannot move a load of C above a
+        // store to C).
+        float cv[FN][16];
+        if (rmw) {
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int col = col0 + 32 * j + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    cv[j][r] = (col < g.N) ? g.C[(int64_t)col * g.ldc + row] : 0.0f;
+                }
+        }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
 #pragma unroll
@@ -333,8 +348,8 @@ __device__ __forceinline__ void store_tile(const GemmArgs &g, int split, int row
                     else ((float *)g.partial)[off] = (float)v;
                 } else if (g.update) {
                     float *c = g.C + (int64_t)col * g.ldc + row;
-                    if (CH64 || g.update == 2) *c = (float)((double)*c + (double)g.alpha * v);
-                    else *c = fmaf(g.alpha, (float)v, *c);
+                    if (CH64 || g.update == 2) *c = (float)((double)cv[j][r] + (double)g.alpha * v);
+                    else *c = fmaf(g.alpha, (float)v, cv[j][r]);
                 } else {
                     g.C[(int64_t)col * g.ldc + row] = (float)v;
                 }
