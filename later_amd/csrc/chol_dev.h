@@ -198,11 +198,17 @@ __device__ __forceinline__ bool cholesky_wave(const double *S, const double *d0,
     double g[kW];
 #pragma unroll
     for (int i = 0; i < kW; ++i) g[i] = aug ? (i == j ? 1.0 : 0.0) : S[i * LDS_ + j];
+    // The reference norms in registers before the elimination: read inside
+    // the loop, each would put an LDS latency on the serial chain (the loop's
+    // stores to R and Ri keep the compiler from hoisting them).
+    double dk[kW];
+#pragma unroll
+    for (int k = 0; k < kW; ++k) dk[k] = d0[k];
     bool bad = false;
 #pragma unroll
     for (int k = 0; k < kW; ++k) {
         const double p = readlane_f64(g[k], k);
-        const double p0 = d0[k];
+        const double p0 = dk[k];
         const bool ok = p0 > 0.0 && p > kPivotRel * p0 && p <= 1.7976931348623157e308;
         bad |= !ok;
         // 1/sqrt(p): hardware estimate and two Newton steps.

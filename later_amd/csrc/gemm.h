@@ -80,6 +80,11 @@ int gemm_pieces(int prec);
 int gemm_split(hipStream_t s, int prec, int rows, int cols, const float *X, int64_t ldx, int a_exp,
                const int *col_exp, uint16_t *out, int64_t ldo, int64_t stride, bool trans, int *launches);
 
+// The transposed split writing exactly `cols` columns of `out` (no zero
+// padding): a column range of a wider array, e.g. persistent pieces of Q^T.
+int gemm_split_t_exact(hipStream_t s, int prec, int rows, int cols, const float *X, int64_t ldx, int a_exp,
+                       uint16_t *out, int64_t ldo, int64_t stride, int *launches);
+
 // gemm_colexp and gemm_split (not transposed) of the same X in one launch:
 // col_exp[j] for each column and the pieces scaled by 2^col_exp[j]
 // (scaled precisions only).

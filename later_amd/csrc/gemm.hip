@@ -1164,9 +1164,13 @@ __global__ __launch_bounds__(256) void split_exp_kernel(int rows, const float *X
 // Transposed pieces: element (i, j) of X goes to out + i * ldo + j.  64 x 64
 // tiles through LDS, so both the FP32 reads (along i) and the 16-bit writes
 // (along j) are contiguous.
+// jw: the columns written, cols rounded up to 64 (zero K padding) or cols
+// exactly (a column range of a wider persistent array: nothing past it is
+// touched).
 template <int PREC>
 __global__ __launch_bounds__(256) void split_t_kernel(int rows, int cols, const float *X, int64_t ldx, int a_exp,
-                                                     const int *col_exp, uint16_t *out, int64_t ldo, int64_t stride)
+                                                     const int *col_exp, uint16_t *out, int64_t ldo, int64_t stride,
+                                                     int jw)
 {
     typedef typename Lds<PREC>::T T;
     constexpr int NP = pieces<PREC>();
@@ -1193,23 +1197,32 @@ __global__ __launch_bounds__(256) void split_t_kernel(int rows, int cols, const 
 #pragma unroll
             for (int c = 0; c < NP; ++c) pv[c][u >> 3][u & 7] = __builtin_bit_cast(uint16_t, pc[c]);
         }
+        if (j0 + jq + 16 <= jw) {
 #pragma unroll
-        for (int c = 0; c < NP; ++c) {
-            ushort8 *dst = (ushort8 *)(out + c * stride + (int64_t)i * ldo + j0 + jq);
-            dst[0] = pv[c][0];
-            dst[1] = pv[c][1];
+            for (int c = 0; c < NP; ++c) {
+                ushort8 *dst = (ushort8 *)(out + c * stride + (int64_t)i * ldo + j0 + jq);
+                dst[0] = pv[c][0];
+                dst[1] = pv[c][1];
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                if (j0 + jq + u < jw)
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) out[c * stride + (int64_t)i * ldo + j0 + jq + u] = pv[c][u >> 3][u & 7];
         }
     }
 }
 
 template <int PREC>
 void launch_split(hipStream_t s, int rows, int cols, const float *X, int64_t ldx, int a_exp, const int *col_exp,
-                  uint16_t *out, int64_t ldo, int64_t stride, bool trans)
+                  uint16_t *out, int64_t ldo, int64_t stride, bool trans, bool pad = true)
 {
     if (trans) {
         dim3 grid((unsigned)ceil_div64(rows, 64), (unsigned)ceil_div64(cols, 64));
+        const int jw = pad ? (int)(ceil_div64(cols, 64) * 64) : cols;
         hipLaunchKernelGGL(split_t_kernel<PREC>, grid, dim3(256), 0, s, rows, cols, X, ldx, a_exp, col_exp, out, ldo,
-                           stride);
+                           stride, jw);
     } else {
         dim3 grid((unsigned)ceil_div64((rows + 63) / 64 * 64, 2048), (unsigned)cols);
         hipLaunchKernelGGL(split_kernel<PREC>, grid, dim3(256), 0, s, rows, cols, X, ldx, a_exp, col_exp, out, ldo,
@@ -1530,6 +1543,22 @@ int gemm_split(hipStream_t s, int prec, int rows, int cols, const float *X, int6
     case GEMM_BF16: launch_split<GEMM_BF16>(s, rows, cols, X, ldx, a_exp, col_exp, out, ldo, stride, trans); break;
     case GEMM_F16X3: launch_split<GEMM_F16X3>(s, rows, cols, X, ldx, a_exp, col_exp, out, ldo, stride, trans); break;
     case GEMM_F16X6: launch_split<GEMM_F16X6>(s, rows, cols, X, ldx, a_exp, col_exp, out, ldo, stride, trans); break;
+    default: return LATER_ERR_OPTION;
+    }
+    HIP_OK(hipGetLastError());
+    if (launches) ++*launches;
+    return LATER_OK;
+}
+
+int gemm_split_t_exact(hipStream_t s, int prec, int rows, int cols, const float *X, int64_t ldx, int a_exp,
+                       uint16_t *out, int64_t ldo, int64_t stride, int *launches)
+{
+    if (rows <= 0 || cols <= 0) return LATER_OK;
+    switch (prec) {
+    case GEMM_F16: launch_split<GEMM_F16>(s, rows, cols, X, ldx, a_exp, nullptr, out, ldo, stride, true, false); break;
+    case GEMM_BF16: launch_split<GEMM_BF16>(s, rows, cols, X, ldx, a_exp, nullptr, out, ldo, stride, true, false); break;
+    case GEMM_F16X3: launch_split<GEMM_F16X3>(s, rows, cols, X, ldx, a_exp, nullptr, out, ldo, stride, true, false); break;
+    case GEMM_F16X6: launch_split<GEMM_F16X6>(s, rows, cols, X, ldx, a_exp, nullptr, out, ldo, stride, true, false); break;
     default: return LATER_ERR_OPTION;
     }
     HIP_OK(hipGetLastError());

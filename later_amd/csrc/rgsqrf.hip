@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "common.h"
 #include "gemm.h"
@@ -48,6 +49,8 @@ int pass_nmin_u(const later_opts &o, bool second) { return second ? std::max(o.n
 struct Layout {
     int64_t header, exps, panel, gemm, snode, r2, tcopy, total;
     int64_t hpieces;  // 16-bit elements of `hwork` for the pre-split operand pieces
+    int64_t hrow;     // offset of Q's transposed pieces in hwork
+    int64_t hnode;    // offset of the per-node operand pieces in hwork
 };
 
 // Nodes at least this wide stage their 16-bit GEMM operands as pre-split
@@ -62,19 +65,29 @@ int64_t piece_ld(int64_t x) { return (x + 7) / 8 * 8 + 64; }
 // Pieces of the projection operands.
 int proj_pieces(int prec) { return gemm_pieces(prec); }
 
-// 16-bit elements the pre-split operands of one node need: the e4m3 bytes
-// of Q1 and A2 for FP8 projections (any width above nmin); for nodes of
-// kPieceMinN columns and more, Q1 and A2 as 16-bit pieces for the projection
-// and Q1 (transposed) and R12 for the update.
-int64_t node_pieces(int m, int n, int p, int pu)
+// Regions of hwork are multiples of 128 elements (256 bytes).
+int64_t up128(int64_t x) { return (x + 127) / 128 * 128; }
+
+// Does a pass whose nodes run projections `p` and updates `pu` stage its
+// operands as 16-bit pieces (at nodes of kPieceMinN columns and more)?
+bool pass_pieces(int n, int nmin_p, int p, int pu)
+{
+    return n >= kPieceMinN && n > nmin_p && p != LATER_GEMM_F32 && gemm_pieces(pu) > 0;
+}
+
+// 16-bit elements the per-node staged operands of one node need: the e4m3
+// bytes of Q1 and A2 for FP8 projections (any width above nmin); for nodes
+// of kPieceMinN columns and more, A2 as 16-bit pieces for the projection and
+// R12 for the update (Q's pieces persist across the nodes of a pass).
+int64_t node_scratch(int m, int n, int p, int pu)
 {
     const int n1 = split_n(n), n2 = n - n1;
     const int64_t ldm = piece_ld(m), ldn1 = piece_ld(n1);
     int64_t tot = 0;
     if (p == LATER_GEMM_FP8) tot += (fp8_bytes(m, n1) + fp8_bytes(m, n2)) / 2;
     if (n >= kPieceMinN && gemm_pieces(pu) > 0) {
-        if (p != LATER_GEMM_FP8) tot += proj_pieces(p) * ldm * (n1 + n2);
-        tot += gemm_pieces(pu) * ldn1 * ((int64_t)m + n2);
+        if (p != LATER_GEMM_FP8) tot += proj_pieces(p) * ldm * n2;
+        tot += gemm_pieces(pu) * ldn1 * n2;
     }
     return tot;
 }
@@ -106,12 +119,23 @@ Layout layout(int m, int n, const later_opts &o)
     L.r2 = (o.reorth & LATER_REORTH_PASS) ? (int64_t)n * n : 0;
     L.tcopy = L.r2;
     L.total = L.header + L.exps + L.panel + L.gemm + L.snode + L.r2 + L.tcopy;
-    // The top node is the largest.
-    int64_t ph = (o.gemm != LATER_GEMM_F32 && n > o.nmin) ? node_pieces(m, n, o.gemm, update_of(o, o.gemm)) : 0;
-    if ((o.reorth & LATER_REORTH_PASS) && o.gemm_pass2 != LATER_GEMM_F32 && n > pass_nmin_p(o, true)) {
-        const int64_t p2 = node_pieces(m, n, o.gemm_pass2, update_of(o, o.gemm_pass2));
-        if (p2 > ph) ph = p2;
+    // hwork: Q's persistent pieces (column-major for the projections,
+    // transposed for the updates; written once per column and pass), then
+    // the largest node's own staged operands (the top node's).
+    int64_t hcol = 0, hrow = 0, hnode = 0;
+    for (int pass = 0; pass < ((o.reorth & LATER_REORTH_PASS) ? 2 : 1); ++pass) {
+        const int p = pass ? o.gemm_pass2 : o.gemm, pu = update_of(o, p);
+        const int nmin_p = pass_nmin_p(o, pass == 1);
+        if (p == LATER_GEMM_F32 || n <= nmin_p) continue;
+        hnode = std::max(hnode, node_scratch(m, n, p, pu));
+        if (pass_pieces(n, nmin_p, p, pu)) {
+            if (p != LATER_GEMM_FP8) hcol = std::max(hcol, proj_pieces(p) * piece_ld(m) * n);
+            hrow = std::max(hrow, gemm_pieces(pu) * piece_ld(n) * m);
+        }
     }
+    L.hrow = up128(hcol);
+    L.hnode = L.hrow + up128(hrow);
+    int64_t ph = hnode > 0 ? L.hnode + hnode : 0;
     // The final R = R1 + (R2 - I) R1 stages R2 (transposed) and R1.
     if ((o.reorth & LATER_REORTH_PASS) && o.gemm_pass2 != LATER_GEMM_F32 && n >= kPieceMinN) {
         const int64_t pt = 2 * gemm_pieces(update_of(o, o.gemm_pass2)) * piece_ld(n) * n;
@@ -130,6 +154,11 @@ struct Run {
     float *gemm; int64_t gemm_n;
     float *snode;
     uint16_t *pcs;     // pre-split operand pieces (nullptr: fused conversion)
+    uint16_t *qcol;    // Q's pieces, column-major (ld ldqc, piece stride qcol_ps)
+    uint16_t *qrow;    // Q's pieces, transposed (ld ldqr, piece stride qrow_ps)
+    uint16_t *npcs;    // the current node's own staged operands
+    int64_t ldqc, ldqr, qcol_ps, qrow_ps;
+    std::vector<unsigned char> col_done, row_done;  // columns of Q split in this pass
     int launches;
     int depth;
     bool node_reorth;  // block CGS2 at the nodes of this pass
@@ -182,7 +211,32 @@ __global__ void copy_upper_kernel(int n, const float *X, int64_t ldx, float *Y, 
 
 dim3 grid1(int64_t n) { return dim3((unsigned)ceil_div64(n, 256)); }
 
-int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, int prec, int depth)
+// Q's columns [c0, c0 + cols) (the node's Q1, FP32 at A) in the persistent
+// pieces of this pass: the column ranges not split yet are split now.  A
+// column of Q is final once its leaf has run, so each is split at most once
+// per pass, by the first node whose Q1 holds it (the pieces are the bits the
+// per-node split would give: Q is scaled by the fixed 2^aexp).
+template <bool ROW>
+int ensure_q(Run &r, int prec, int m, int c0, int cols, const float *A, int64_t lda, int aexp)
+{
+    std::vector<unsigned char> &done = ROW ? r.row_done : r.col_done;
+    int rc = LATER_OK;
+    for (int a = c0; a < c0 + cols && !rc;) {
+        if (done[a]) { ++a; continue; }
+        int b = a;
+        while (b < c0 + cols && !done[b]) done[b++] = 1;
+        const float *X = A + (int64_t)(a - c0) * lda;
+        if (ROW)
+            rc = gemm_split_t_exact(r.s, prec, m, b - a, X, lda, aexp, r.qrow + a, r.ldqr, r.qrow_ps, &r.launches);
+        else
+            rc = gemm_split(r.s, prec, m, b - a, X, lda, aexp, nullptr, r.qcol + (int64_t)a * r.ldqc, r.ldqc,
+                            r.qcol_ps, false, &r.launches);
+        a = b;
+    }
+    return rc;
+}
+
+int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, int prec, int depth, int c0)
 {
     if (depth > r.depth) r.depth = depth;
     // Cholesky QR leaves take up to 64 columns (one Gram and factor for what
@@ -200,7 +254,7 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     const int pu = inner_u ? LATER_GEMM_F32 : update_of(r.o, prec);  // update precision
     const int chunk = inner ? 0 : r.chunk;
     const bool f64 = r.chunk > 0 && !inner_u;  // split updates subtract in FP64 when the pass folds
-    int rc = qr_rec(r, m, n1, A, lda, R, ldr, prec, depth + 1);
+    int rc = qr_rec(r, m, n1, A, lda, R, ldr, prec, depth + 1, c0);
     if (rc) return rc;
     float *A2 = A + (int64_t)n1 * lda;
     float *R12 = R + (int64_t)n1 * ldr;
@@ -216,7 +270,7 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     if (fp8 && !pre_proj) return LATER_ERR_WORKSPACE;
     Pieces q1h{}, a2h{}, q1t{}, r12h{};
     {
-        uint16_t *b = r.pcs;
+        uint16_t *b = r.npcs;
         if (fp8) {
             q1h = fp8_pieces(b, m, n1);
             b += fp8_bytes(m, n1) / 2;
@@ -224,12 +278,12 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
             b += fp8_bytes(m, n2) / 2;
         } else if (pre_proj) {
             const int64_t ldm = piece_ld(m);
-            q1h = {b, ldm, ldm * n1}; b += proj_pieces(p) * ldm * n1;
+            q1h = {r.qcol + (int64_t)c0 * r.ldqc, r.ldqc, r.qcol_ps};
             a2h = {b, ldm, ldm * n2}; b += proj_pieces(p) * ldm * n2;
         }
         if (pre) {
             const int64_t ldn1 = piece_ld(n1);
-            q1t = {b, ldn1, ldn1 * m}; b += gemm_pieces(pu) * ldn1 * m;
+            q1t = {r.qrow + c0, r.ldqr, r.qrow_ps};
             r12h = {b, ldn1, ldn1 * n2};
         }
     }
@@ -239,9 +293,7 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
         if ((rc = gemm_split8(r.s, m, n1, A, lda, (void *)q1h.p, &r.launches))) return rc;
         if ((rc = gemm_split8(r.s, m, n2, A2, lda, (void *)a2h.p, &r.launches))) return rc;
     } else {
-        if (pre_proj && (rc = gemm_split(r.s, p, m, n1, A, lda, aexp, nullptr, (uint16_t *)q1h.p, q1h.ld,
-                                         q1h.stride, false, &r.launches)))
-            return rc;
+        if (pre_proj && (rc = ensure_q<false>(r, p, m, c0, n1, A, lda, aexp))) return rc;
         if (pre_proj && scaled(p)) {
             if ((rc = gemm_split_exp(r.s, p, m, n2, A2, lda, r.exps, (uint16_t *)a2h.p, a2h.ld, a2h.stride,
                                      &r.launches)))
@@ -259,9 +311,7 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     // A2 -= Q1 R12    (QR/later_rgsqrf.cu:51-56); R12's column exponents
     // replace A2's.
     if (pre) {
-        if ((rc = gemm_split(r.s, pu, m, n1, A, lda, aexp_u, nullptr, (uint16_t *)q1t.p, q1t.ld, q1t.stride, true,
-                             &r.launches)))
-            return rc;
+        if ((rc = ensure_q<true>(r, pu, m, c0, n1, A, lda, aexp_u))) return rc;
         if (scaled(pu))
             rc = gemm_split_exp(r.s, pu, n1, n2, R12, ldr, r.exps, (uint16_t *)r12h.p, r12h.ld, r12h.stride,
                                 &r.launches);
@@ -294,7 +344,7 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
         HIP_OK(hipGetLastError());
         ++r.launches;
     }
-    return qr_rec(r, m, n2, A2, lda, R + n1 + (int64_t)n1 * ldr, ldr, prec, depth + 1);
+    return qr_rec(r, m, n2, A2, lda, R + n1 + (int64_t)n1 * ldr, ldr, prec, depth + 1, c0 + n1);
 }
 
 bool overlap(const void *a, int64_t abytes, const void *b, int64_t bbytes)
@@ -437,6 +487,15 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     float *R2 = p; p += L.r2;
     float *T = p; p += L.tcopy;
     r.pcs = with_pieces ? hwork : nullptr;
+    r.qcol = r.pcs;
+    r.qrow = r.pcs ? r.pcs + L.hrow : nullptr;
+    r.npcs = r.pcs ? r.pcs + L.hnode : nullptr;
+    r.ldqc = piece_ld(m);
+    r.ldqr = piece_ld(n);
+    r.qcol_ps = r.ldqc * n;
+    r.qrow_ps = r.ldqr * m;
+    r.col_done.assign(n, 0);
+    r.row_done.assign(n, 0);
     r.launches = 0;
     r.depth = 0;
     r.node_reorth = (o.reorth & LATER_REORTH_NODE) != 0;
@@ -449,7 +508,7 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     hipLaunchKernelGGL(zero_lower_kernel, grid_cols(n, n), dim3(256), 0, r.s, n, R, (int64_t)ldr);
     r.launches += 2;
     rc = hipGetLastError() == hipSuccess ? LATER_OK : LATER_ERR_DEVICE;
-    if (!rc) rc = qr_rec(r, m, n, A, lda, R, ldr, o.gemm, 0);
+    if (!rc) rc = qr_rec(r, m, n, A, lda, R, ldr, o.gemm, 0, 0);
     if (!rc && (o.reorth & LATER_REORTH_PASS)) {
         // Second pass on Q, then R = R1 + (R2 - I) R1 (upper-triangular product).
         hipLaunchKernelGGL(fill_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, (int64_t)n * n, R2, 0.0f);
@@ -459,7 +518,9 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
         r.nmin_p = pass_nmin_p(o, true);
         r.nmin_u = pass_nmin_u(o, true);
         r.chunk = o.chunk2 > 0 ? o.chunk2 : o.chunk;
-        rc = qr_rec(r, m, n, A, lda, R2, n, o.gemm_pass2, 0);
+        r.col_done.assign(n, 0);  // the second pass writes a new Q
+        r.row_done.assign(n, 0);
+        rc = qr_rec(r, m, n, A, lda, R2, n, o.gemm_pass2, 0, 0);
         if (!rc) {
             hipLaunchKernelGGL(copy_upper_kernel, grid_cols(n, n), dim3(256), 0, r.s, n, R, (int64_t)ldr, T,
                                0.0f);
