@@ -85,6 +85,11 @@ int gemm_split(hipStream_t s, int prec, int rows, int cols, const float *X, int6
 int gemm_split_t_exact(hipStream_t s, int prec, int rows, int cols, const float *X, int64_t ldx, int a_exp,
                        uint16_t *out, int64_t ldo, int64_t stride, int *launches);
 
+// The transposed split (zero-padded like gemm_split) of X's upper triangle
+// (n x n) with `shift` subtracted from the diagonal and zeros below it.
+int gemm_split_t_tri(hipStream_t s, int prec, int n, const float *X, int64_t ldx, int a_exp, float shift,
+                     uint16_t *out, int64_t ldo, int64_t stride, int *launches);
+
 // gemm_colexp and gemm_split (not transposed) of the same X in one launch:
 // col_exp[j] for each column and the pieces scaled by 2^col_exp[j]
 // (scaled precisions only).

@@ -1166,11 +1166,12 @@ __global__ __launch_bounds__(256) void split_exp_kernel(int rows, const float *X
 // (along j) are contiguous.
 // jw: the columns written, cols rounded up to 64 (zero K padding) or cols
 // exactly (a column range of a wider persistent array: nothing past it is
-// touched).
+// touched).  tri: X is read as its upper triangle minus `shift` on the
+// diagonal (zeros below), e.g. R2 - I without a copy of it.
 template <int PREC>
 __global__ __launch_bounds__(256) void split_t_kernel(int rows, int cols, const float *X, int64_t ldx, int a_exp,
                                                      const int *col_exp, uint16_t *out, int64_t ldo, int64_t stride,
-                                                     int jw)
+                                                     int jw, int tri, float shift)
 {
     typedef typename Lds<PREC>::T T;
     constexpr int NP = pieces<PREC>();
@@ -1181,7 +1182,11 @@ __global__ __launch_bounds__(256) void split_t_kernel(int rows, int cols, const 
     for (int q = 0; q < 16; ++q) {
         const int jj = ty + 4 * q, i = i0 + tx, j = j0 + jj;
         float v = 0.0f;
-        if (i < rows && j < cols) v = X[i + (int64_t)j * ldx] * ldexpf(1.0f, col_exp ? col_exp[j] : a_exp);
+        if (i < rows && j < cols && !(tri && i > j)) {
+            float x = X[i + (int64_t)j * ldx];
+            if (tri && i == j) x -= shift;
+            v = x * ldexpf(1.0f, col_exp ? col_exp[j] : a_exp);
+        }
         tile[jj][tx] = v;
     }
     __syncthreads();
@@ -1216,13 +1221,14 @@ __global__ __launch_bounds__(256) void split_t_kernel(int rows, int cols, const 
 
 template <int PREC>
 void launch_split(hipStream_t s, int rows, int cols, const float *X, int64_t ldx, int a_exp, const int *col_exp,
-                  uint16_t *out, int64_t ldo, int64_t stride, bool trans, bool pad = true)
+                  uint16_t *out, int64_t ldo, int64_t stride, bool trans, bool pad = true, bool tri = false,
+                  float shift = 0.0f)
 {
     if (trans) {
         dim3 grid((unsigned)ceil_div64(rows, 64), (unsigned)ceil_div64(cols, 64));
         const int jw = pad ? (int)(ceil_div64(cols, 64) * 64) : cols;
         hipLaunchKernelGGL(split_t_kernel<PREC>, grid, dim3(256), 0, s, rows, cols, X, ldx, a_exp, col_exp, out, ldo,
-                           stride, jw);
+                           stride, jw, tri ? 1 : 0, shift);
     } else {
         dim3 grid((unsigned)ceil_div64((rows + 63) / 64 * 64, 2048), (unsigned)cols);
         hipLaunchKernelGGL(split_kernel<PREC>, grid, dim3(256), 0, s, rows, cols, X, ldx, a_exp, col_exp, out, ldo,
@@ -1559,6 +1565,22 @@ int gemm_split_t_exact(hipStream_t s, int prec, int rows, int cols, const float 
     case GEMM_BF16: launch_split<GEMM_BF16>(s, rows, cols, X, ldx, a_exp, nullptr, out, ldo, stride, true, false); break;
     case GEMM_F16X3: launch_split<GEMM_F16X3>(s, rows, cols, X, ldx, a_exp, nullptr, out, ldo, stride, true, false); break;
     case GEMM_F16X6: launch_split<GEMM_F16X6>(s, rows, cols, X, ldx, a_exp, nullptr, out, ldo, stride, true, false); break;
+    default: return LATER_ERR_OPTION;
+    }
+    HIP_OK(hipGetLastError());
+    if (launches) ++*launches;
+    return LATER_OK;
+}
+
+int gemm_split_t_tri(hipStream_t s, int prec, int n, const float *X, int64_t ldx, int a_exp, float shift,
+                     uint16_t *out, int64_t ldo, int64_t stride, int *launches)
+{
+    if (n <= 0) return LATER_OK;
+    switch (prec) {
+    case GEMM_F16: launch_split<GEMM_F16>(s, n, n, X, ldx, a_exp, nullptr, out, ldo, stride, true, true, true, shift); break;
+    case GEMM_BF16: launch_split<GEMM_BF16>(s, n, n, X, ldx, a_exp, nullptr, out, ldo, stride, true, true, true, shift); break;
+    case GEMM_F16X3: launch_split<GEMM_F16X3>(s, n, n, X, ldx, a_exp, nullptr, out, ldo, stride, true, true, true, shift); break;
+    case GEMM_F16X6: launch_split<GEMM_F16X6>(s, n, n, X, ldx, a_exp, nullptr, out, ldo, stride, true, true, true, shift); break;
     default: return LATER_ERR_OPTION;
     }
     HIP_OK(hipGetLastError());

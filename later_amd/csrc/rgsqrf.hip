@@ -510,9 +510,9 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     rc = hipGetLastError() == hipSuccess ? LATER_OK : LATER_ERR_DEVICE;
     if (!rc) rc = qr_rec(r, m, n, A, lda, R, ldr, o.gemm, 0, 0);
     if (!rc && (o.reorth & LATER_REORTH_PASS)) {
-        // Second pass on Q, then R = R1 + (R2 - I) R1 (upper-triangular product).
-        hipLaunchKernelGGL(fill_kernel, grid1((int64_t)n * n), dim3(256), 0, r.s, (int64_t)n * n, R2, 0.0f);
-        ++r.launches;
+        // Second pass on Q, then R = R1 + (R2 - I) R1 (upper-triangular
+        // product).  R2's blocks below the diagonal are never written: its
+        // products read the upper triangle only.
         r.node_reorth = false;  // Q has condition number ~1: one projection per node suffices
         r.chol_once = true;     // and one Cholesky QR pass per leaf
         r.nmin_p = pass_nmin_p(o, true);
@@ -522,27 +522,30 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
         r.row_done.assign(n, 0);
         rc = qr_rec(r, m, n, A, lda, R2, n, o.gemm_pass2, 0, 0);
         if (!rc) {
-            hipLaunchKernelGGL(copy_upper_kernel, grid_cols(n, n), dim3(256), 0, r.s, n, R, (int64_t)ldr, T,
-                               0.0f);
-            hipLaunchKernelGGL(copy_upper_kernel, grid_cols(n, n), dim3(256), 0, r.s, n, R2, (int64_t)n, R2,
-                               1.0f);
-            r.launches += 2;
             const int pu = update_of(o, o.gemm_pass2);
             const int *bexp = scaled(pu) ? r.exps : nullptr;
             const int aexp = scaled(pu) ? kQExp : 0;
             const bool pre = r.pcs && n >= kPieceMinN && gemm_pieces(pu) > 0;
             Pieces rt{}, tt{};
             if (pre) {
+                // Staged operands straight from R2 and R (R1, zero below its
+                // diagonal): R2 - I is formed in the split, and the product
+                // reads only the pieces, so C = R is safe.
                 const int64_t ld = piece_ld(n);
                 rt = {r.pcs, ld, ld * n};
                 tt = {r.pcs + gemm_pieces(pu) * ld * n, ld, ld * n};
-                rc = gemm_split(r.s, pu, n, n, R2, n, aexp, nullptr, (uint16_t *)rt.p, ld, ld * n, true, &r.launches);
+                rc = gemm_split_t_tri(r.s, pu, n, R2, n, aexp, 1.0f, (uint16_t *)rt.p, ld, ld * n, &r.launches);
                 if (!rc && scaled(pu))
-                    rc = gemm_split_exp(r.s, pu, n, n, T, n, r.exps, (uint16_t *)tt.p, ld, ld * n, &r.launches);
+                    rc = gemm_split_exp(r.s, pu, n, n, R, ldr, r.exps, (uint16_t *)tt.p, ld, ld * n, &r.launches);
                 else if (!rc)
-                    rc = gemm_split(r.s, pu, n, n, T, n, 0, bexp, (uint16_t *)tt.p, ld, ld * n, false, &r.launches);
-            } else if (scaled(pu)) {
-                rc = gemm_colexp(r.s, n, n, T, n, r.exps, &r.launches);
+                    rc = gemm_split(r.s, pu, n, n, R, ldr, 0, bexp, (uint16_t *)tt.p, ld, ld * n, false, &r.launches);
+            } else {
+                hipLaunchKernelGGL(copy_upper_kernel, grid_cols(n, n), dim3(256), 0, r.s, n, R, (int64_t)ldr, T,
+                                   0.0f);
+                hipLaunchKernelGGL(copy_upper_kernel, grid_cols(n, n), dim3(256), 0, r.s, n, R2, (int64_t)n, R2,
+                                   1.0f);
+                r.launches += 2;
+                if (scaled(pu)) rc = gemm_colexp(r.s, n, n, T, n, r.exps, &r.launches);
             }
             if (!rc)
                 rc = gemm_nn(r.s, pu, n, n, n, 1.0f, R2, n, T, n, R, ldr, o.chunk > 0, bexp, aexp, true,
