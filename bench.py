@@ -296,6 +296,38 @@ def batched_config(later, torch, count=4096, steps=3):
     return out
 
 
+def out_of_core_config(later, torch, m=65536, n=8192, budget=1 << 30, forms=("reference", "mixed")):
+    """later_oc_qr_rec on a 65536 x 8192 N(0,1) matrix (2 GiB) in pinned host
+    memory with a 1 GiB device budget: wall time of the synchronous call,
+    host-device copies included (the rate is PCIe-inclusive), and the FP64
+    metrics of the result."""
+    out = {"m": m, "n": n, "device_budget_bytes": budget, "input": "normal seed 0",
+           "timing": "wall clock of the synchronous call, host<->device copies included"}
+    A0 = later.generate_normal(m, n, seed=0, mat=0)
+    host = torch.empty((n, m), dtype=torch.float32, pin_memory=True).t()
+    Rh = torch.empty((n, n), dtype=torch.float32, pin_memory=True).t()
+    for form in forms:
+        o = later.opts(form)
+        host.copy_(A0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        later.oc_qr_rec(host, Rh, budget, o)
+        ms = (time.perf_counter() - t0) * 1e3
+        Q = later.colmajor(m, n)
+        Q.copy_(host)
+        R = later.colmajor(n, n)
+        R.copy_(Rh)
+        torch.cuda.synchronize()
+        e = {"ms": ms, "tflops_ref": later.flops_ref(m, n) / ms / 1e9, "orth": later.check_orth(Q),
+             "backward": later.check_backward(A0, Q, R)}
+        out[form] = e
+        log(f"[bench] out-of-core {m}x{n} ({budget >> 20} MiB device) {form}: {ms:.1f} ms  "
+            f"{e['tflops_ref']:.1f} TFLOP/s(ref, PCIe-inclusive)  orth={e['orth']:.2e} bwd={e['backward']:.2e}")
+        del Q, R
+        torch.cuda.empty_cache()
+    return out
+
+
 def serving_config(later, torch, count=64, rates=(100.0, 1e9), slots=8, budget=16 << 30):
     """Least-squares serving: requests with m log-uniform in [128, 32768],
     n log-uniform in [16, 1024] (n <= m), condition number log-uniform in
@@ -561,6 +593,7 @@ def main():
             "C3b": single_config(later, torch, "C3b", 262144, 256, "normal", ("reference", "mixed", "fp8", "bf16")),
             "C4": batched_config(later, torch),
             "C5": serving_config(later, torch),
+            "out_of_core": out_of_core_config(later, torch),
         }
 
     cpu = None

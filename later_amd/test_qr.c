@@ -13,6 +13,11 @@
  *                                the reference's workspace sizes, :73-78)
  *   -normal                      N(0,1) entries instead of 2u - 1
  *   -seed S                      generator seed (default 0)
+ *   -oc MIB                      out of core, as test/test_oc_qr.cu:69-75 drives
+ *                                later_oc_qr_rec: A and R in pinned host memory,
+ *                                at most MIB MiB of device memory (options of
+ *                                -form, default reference); the time includes
+ *                                the host-device copies
  * The last line is machine-readable:
  *   result algo=1 m=.. n=.. form=.. ms=.. gflops=.. orth=.. orth_over_n=.. backward=.. status=..
  * Exit status: 0 on success, 1 on a library or runtime error, 2 on bad usage.
@@ -39,7 +44,7 @@ static void usage(void)
 {
     printf("Usage: test_qr algo m n [options]\n");
     printf("Options:\n\t-check: enable checking the orthogonality and backward error\n");
-    printf("\t-form reference|fp32|mixed\n\t-normal\n\t-seed S\n");
+    printf("\t-form reference|fp32|mixed\n\t-normal\n\t-seed S\n\t-oc MIB\n");
 }
 
 int main(int argc, char **argv)
@@ -52,6 +57,7 @@ int main(int argc, char **argv)
     const int m = atoi(argv[2]);
     const int n = atoi(argv[3]);
     int check = 0, normal = 0;
+    long oc_mib = 0;
     unsigned seed = 0;
     const char *form = NULL;
     for (int i = 4; i < argc; ++i) {
@@ -59,6 +65,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "-normal")) normal = 1;
         else if (!strcmp(argv[i], "-seed") && i + 1 < argc) seed = (unsigned)strtoul(argv[++i], NULL, 10);
         else if (!strcmp(argv[i], "-form") && i + 1 < argc) form = argv[++i];
+        else if (!strcmp(argv[i], "-oc") && i + 1 < argc) oc_mib = strtol(argv[++i], NULL, 10);
         else {
             usage();
             return 2;
@@ -95,6 +102,63 @@ int main(int argc, char **argv)
     if (rc) {
         fprintf(stderr, "generator status %d\n", rc);
         return 1;
+    }
+
+    if (oc_mib > 0) {
+        /* Out of core: the matrix goes to pinned host memory and comes back
+         * factored; the device holds at most oc_mib MiB meanwhile. */
+        if (!form) later_opts_default(&o);
+        float *hA = NULL, *hR = NULL;
+        CHECK_HIP(hipHostMalloc((void **)&hA, sizeof(float) * (size_t)m * n, 0));
+        CHECK_HIP(hipHostMalloc((void **)&hR, sizeof(float) * (size_t)n * n, 0));
+        CHECK_HIP(hipMemcpyAsync(hA, A, sizeof(float) * (size_t)m * n, hipMemcpyDeviceToHost, s));
+        CHECK_HIP(hipStreamSynchronize(s));
+        printf("Perform OC_QR_REC\nmatrix size %d*%d, device budget %ld MiB\n", m, n, oc_mib);
+        hipEvent_t f0, f1;
+        CHECK_HIP(hipEventCreate(&f0));
+        CHECK_HIP(hipEventCreate(&f1));
+        CHECK_HIP(hipEventRecord(f0, s));
+        later_info info;
+        const int st = later_oc_qr_rec(ctxt, m, n, hA, m, hR, n, (int64_t)oc_mib << 20, &o, &info);
+        CHECK_HIP(hipEventRecord(f1, s));
+        CHECK_HIP(hipEventSynchronize(f1));
+        float ms = 0.0f;
+        CHECK_HIP(hipEventElapsedTime(&ms, f0, f1));
+        if (st) {
+            fprintf(stderr, "later_oc_qr_rec status %d\n", st);
+            return 1;
+        }
+        const double gflops = 2.0 * n * n * (m - n / 3.0) / (ms * 1e6);
+        printf("OC_QR_REC takes %.3f ms, exec rate %.0f GFLOPS\n", ms, gflops);
+        double orth = -1.0, bwd = -1.0;
+        if (check) {
+            CHECK_HIP(hipMemcpy(A, hA, sizeof(float) * (size_t)m * n, hipMemcpyHostToDevice));
+            CHECK_HIP(hipMemcpy(R, hR, sizeof(float) * (size_t)n * n, hipMemcpyHostToDevice));
+            CHECK_HIP(hipMalloc((void **)&A0, sizeof(float) * (size_t)m * n));
+            rc = normal ? later_generate_normal(ctxt, m, n, A0, m, seed, 0)
+                        : later_generate_uniform(ctxt, m, n, A0, m, seed, 0);
+            if (!rc) rc = later_check_orth(ctxt, m, n, A, m, &orth);
+            if (!rc) rc = later_check_backward(ctxt, m, n, A0, m, A, m, R, n, &bwd);
+            if (rc) {
+                fprintf(stderr, "check status %d\n", rc);
+                return 1;
+            }
+            printf("orthogonality ||I - Q^T Q||_F / n = %.6e (raw %.6e)\n", orth / n, orth);
+            printf("backward error ||A - QR||_F / ||A||_F = %.6e\n", bwd);
+            CHECK_HIP(hipFree(A0));
+        }
+        printf("result algo=1 m=%d n=%d form=%s oc_mib=%ld ms=%.4f gflops=%.2f orth=%.6e orth_over_n=%.6e "
+               "backward=%.6e status=%d\n",
+               m, n, form ? form : "reference", oc_mib, ms, gflops, orth, orth >= 0 ? orth / n : -1.0, bwd,
+               info.status);
+        CHECK_HIP(hipHostFree(hA));
+        CHECK_HIP(hipHostFree(hR));
+        CHECK_HIP(hipFree(A));
+        CHECK_HIP(hipFree(R));
+        CHECK_HIP(hipEventDestroy(f0));
+        CHECK_HIP(hipEventDestroy(f1));
+        CHECK_HIP(hipStreamDestroy(s));
+        return 0;
     }
 
     int64_t lwork = 0, lhwork = 0;

@@ -215,3 +215,23 @@ def test_c_driver_usage(later):
     assert rc == 2 and "only 1" in out
     rc, _, _ = _driver(1, 10, 20)
     assert rc == 2
+
+
+@pytest.mark.parametrize("form", [None, "mixed"])
+def test_c_driver_out_of_core(later, form):
+    """The C driver's out-of-core mode (the role of test/test_oc_qr.cu:69-75):
+    8192 x 2048 (64 MiB) with a 40 MiB device budget through
+    later_oc_qr_rec on pinned host memory; the reference form's metrics
+    match the in-core run's within 2x, the mixed form meets the gates."""
+    m, n = 8192, 2048
+    args = [1, m, n, "-check", "-oc", 40] + (["-form", form] if form else [])
+    rc, res, out = _driver(*args)
+    assert rc == 0, out
+    assert res["status"] == "0" and res["oc_mib"] == "40"
+    orth, bwd = float(res["orth"]), float(res["backward"])
+    if form is None:
+        rc2, res2, out2 = _driver(1, m, n, "-check", "-form", "reference")
+        assert rc2 == 0, out2
+        assert orth <= 2 * float(res2["orth"]) + 1e-6 and bwd <= 2 * float(res2["backward"]) + 1e-7
+    else:
+        assert orth < 1e-5 and bwd < 1e-6
