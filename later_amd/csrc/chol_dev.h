@@ -98,28 +98,49 @@ template <int W> __device__ __forceinline__ void gram_step(Gram<W> &g, const dou
                     __builtin_amdgcn_mfma_f64_16x16x4f64(a[x][p], a[y][p], g.t[tile_index<CT>(x, y)], 0, 0, 0);
 }
 
-// Adds the four waves' Grams in wave order and stores this workgroup's
-// packed partial.  Cb: 4 x NT x 4 x 64 doubles of LDS.
+// Adds the four waves' Grams, (w0 + w2) + (w1 + w3), in two lane-aligned
+// LDS rounds (every wave reads back exactly the slots it wrote: no bank
+// conflicts), and wave 0 stores this workgroup's packed partial from its
+// registers.  Cb: 2 x NT x 4 x 64 doubles of LDS.
 template <int W> __device__ __forceinline__ void gram_finish(const Gram<W> &g, double *Cb, double *part)
 {
     constexpr int NT = Shape<W>::NT, CT = Shape<W>::CT;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    Gram<W> s = g;
+    if (wv >= 2) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+        for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Cb[((wv * NT + t) * 4 + r) * 64 + lane] = g.t[t][r];
+            for (int r = 0; r < 4; ++r) Cb[(((wv - 2) * NT + t) * 4 + r) * 64 + lane] = g.t[t][r];
+    }
     __syncthreads();
-    for (int e = threadIdx.x; e < Shape<W>::P; e += kThreads) {
-        int i, j;
-        unpack(e, i, j);
-        const int t = tile_index<CT>(i >> 4, j >> 4);
-        const int ii = i & 15, jj = j & 15;
-        const int ln = jj + 16 * (ii & 3), r = ii >> 2;
-        double s = Cb[((0 * NT + t) * 4 + r) * 64 + ln];
-        s += Cb[((1 * NT + t) * 4 + r) * 64 + ln];
-        s += Cb[((2 * NT + t) * 4 + r) * 64 + ln];
-        s += Cb[((3 * NT + t) * 4 + r) * 64 + ln];
-        part[e] = s;
+    if (wv < 2) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) s.t[t][r] += Cb[((wv * NT + t) * 4 + r) * 64 + lane];
+    }
+    __syncthreads();
+    if (wv == 1) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Cb[(t * 4 + r) * 64 + lane] = s.t[t][r];
+    }
+    __syncthreads();
+    if (wv == 0) {
+        // Lane l, item r of tile (a, b) holds entry (16 a + l/16 + 4 r, 16 b + l%16).
+#pragma unroll
+        for (int a = 0; a < CT; ++a)
+#pragma unroll
+            for (int b = a; b < CT; ++b) {
+                const int t = tile_index<CT>(a, b);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * a + (lane >> 4) + 4 * r, j = 16 * b + (lane & 15);
+                    if (i <= j) part[j * (j + 1) / 2 + i] = s.t[t][r] + Cb[(t * 4 + r) * 64 + lane];
+                }
+            }
     }
 }
 
@@ -356,7 +377,7 @@ __device__ __forceinline__ void load_block_mfma(const float *A, int64_t lda, int
 template <int W> struct ApplyLds {
     static constexpr int kFactor = (W * Shape<W>::LDG + 2 * W * W + W) * 8;
     static constexpr int kQs = W * (kRows + 4) * 4;
-    static constexpr int kCb = 4 * Shape<W>::NT * 4 * 64 * 8;
+    static constexpr int kCb = 2 * Shape<W>::NT * 4 * 64 * 8;
     static constexpr int kBytes = kFactor > kQs ? (kFactor > kCb ? kFactor : kCb) : (kQs > kCb ? kQs : kCb);
 };
 

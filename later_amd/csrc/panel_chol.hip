@@ -109,7 +109,7 @@ template <int W> struct BlockRegs {
 template <int W>
 __global__ __launch_bounds__(kThreads) void chol_gram_kernel(int m, int w, const float *A, int64_t lda, float *ws)
 {
-    constexpr int kQs = W * (kRows + 4) * 4, kCb = 4 * Shape<W>::NT * 4 * 64 * 8;
+    constexpr int kQs = W * (kRows + 4) * 4, kCb = 2 * Shape<W>::NT * 4 * 64 * 8;
     __shared__ __attribute__((aligned(16))) char smem[kQs > kCb ? kQs : kCb];
     float *Qs = (float *)smem;
     const CholWork cw = carve<W>(ws);
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kThreads) void chol_apply_kernel(int m, int w, floa
     constexpr int CT = Shape<W>::CT, KS = Shape<W>::KS;
     constexpr int kAs = W * kApplyLd * 4;                       // the block, for the product
     constexpr int kQs = (PASS == 1) ? W * (kRows + 4) * 4 : 0;  // the stored Q1 rows, for their Gram
-    constexpr int kCb = (PASS == 1) ? 4 * Shape<W>::NT * 4 * 64 * 8 : 0;
+    constexpr int kCb = (PASS == 1) ? 2 * Shape<W>::NT * 4 * 64 * 8 : 0;
     constexpr int kBytes = (kAs + kQs) > kCb ? (kAs + kQs) : kCb;
     __shared__ __attribute__((aligned(16))) char smem[kBytes];
     float *As = (float *)smem;
