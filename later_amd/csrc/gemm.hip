@@ -568,12 +568,16 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &g, const int tm, const
 
 // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs, each
 // with its own L2: the linear id is renumbered so that every XCD works
-// through a contiguous range, and each range walks the tiles in groups of 8
-// tile-rows (column by column inside a group), so the workgroups resident on
-// one XCD share their A and B tiles in its L2.
+// through a contiguous range, and each range walks the tiles in groups of
+// kGroup tile-rows (column by column inside a group), so the workgroups
+// resident on one XCD share their A and B tiles in its L2.  With 32
+// resident tiles per XCD the group is kGroup rows x 32 / kGroup columns:
+// 8 x 4 for square tiles, 4 x 8 for the 256 x 128 tiles (half the A strips
+// in flight, whose slabs are twice the B slabs).
+template <int kGroup = 8>
 __device__ __forceinline__ void tile_of(int &tm, int &tn, int &split)
 {
-    constexpr int kXcd = 8, kGroup = 8;
+    constexpr int kXcd = 8;
     const int nx = gridDim.x, ny = gridDim.y;
     const int total = nx * ny * gridDim.z;
     int L = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
@@ -819,7 +823,7 @@ template <int PREC, bool CH64, int BM, int WGM>
 __global__ __launch_bounds__(128 * WGM) void gemm_lds_kernel(GemmArgs g)
 {
     int tm, tn, split;
-    if (!(g.tri && tile_of_tri(tm, tn, split))) tile_of(tm, tn, split);
+    if (!(g.tri && tile_of_tri(tm, tn, split))) tile_of<BM == 256 ? 4 : 8>(tm, tn, split);
     gemm_lds_body<PREC, CH64, BM, WGM>(g, tm, tn, split);
 }
 
