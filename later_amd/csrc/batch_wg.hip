@@ -104,28 +104,36 @@ template <> struct WgMfma<LATER_GEMM_F16> : WgMfma16<LATER_GEMM_F16> {};
 
 // A chunk of rows r0 .. r0 + 63 of columns 0 .. NC - 1 (zeros past m and
 // past nc), held in registers between its global loads and its LDS store so
-// that the next chunk's loads are in flight while the current one is used:
-// thread t holds column t / (256 / NC) and PER consecutive rows of it.
+// that the next chunks' loads are in flight while the current one is used:
+// thread t holds row t % 64 of columns t / 64 + 4 e, so each wave load reads
+// 64 consecutive rows of one column (two whole 128-byte lines) and each LDS
+// store writes 64 consecutive words.
 template <int NC> struct Chunk {
-    static constexpr int PER = kKC * NC / kThreads;  // 8 (32 columns), 16 (64), 32 (128)
-    static constexpr int TPC = kKC / PER;            // threads per column
+    static constexpr int PER = NC / 4;  // 8 (32 columns), 16 (64), 32 (128)
+    static_assert(kKC == 64 && kThreads == 256, "one row per lane, four column sets");
     float v[PER];
+    unsigned ok;  // bit e: the value of column set e is inside the matrix
+    // Every load reads an in-bounds address (rows and columns clamped); the
+    // values outside the matrix are replaced by zeros when stored, so no load
+    // sits behind a branch.
     __device__ __forceinline__ void load(const float *A, int64_t lda, int m, int r0, int nc)
     {
-        const int t = threadIdx.x, c = t / TPC, rb = (t % TPC) * PER;
-        const float *src = A + (int64_t)min(c, nc - 1) * lda + r0 + rb;
+        const auto *G = gmem(A);
+        const int t = threadIdx.x, row = r0 + (t & 63), rc = min(row, m - 1);
+        ok = 0u;
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
-            const int row = r0 + rb + e;
-            v[e] = (c < nc && row < m) ? src[e] : 0.0f;
+            const int c = (t >> 6) + 4 * e;
+            v[e] = G[(int64_t)min(c, nc - 1) * lda + rc];
+            ok |= (c < nc && row < m) ? (1u << e) : 0u;
         }
     }
     // Column-major LDS chunk (stride kLDC).
     __device__ __forceinline__ void store(float *C) const
     {
-        const int t = threadIdx.x, c = t / TPC, rb = (t % TPC) * PER;
+        const int t = threadIdx.x;
 #pragma unroll
-        for (int e = 0; e < PER; ++e) C[c * kLDC + rb + e] = v[e];
+        for (int e = 0; e < PER; ++e) C[((t >> 6) + 4 * e) * kLDC + (t & 63)] = ((ok >> e) & 1u) ? v[e] : 0.0f;
     }
 };
 
@@ -136,6 +144,7 @@ template <int NC> struct Chunk {
 __device__ __noinline__ bool wg_cholqr(int m, int w, float *P, int64_t lda, char *lds)
 {
     constexpr int W = 32, CT = 2, KS = 8, LDG = 33;
+    static_assert(kTT == 1, "one 16-row tile per wave and block");
     double *Gs = (double *)lds;
     double *Rs = Gs + W * LDG;
     double *Ris = Rs + W * W;
@@ -145,6 +154,7 @@ __device__ __noinline__ bool wg_cholqr(int m, int w, float *P, int64_t lda, char
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int c = lane & 15;
     const int nblk = (m + kRows - 1) / kRows;
+    auto *G = gmem(P);
     // Gram
     Gram<W> g;
     gram_zero<W>(g);
@@ -155,19 +165,42 @@ __device__ __noinline__ bool wg_cholqr(int m, int w, float *P, int64_t lda, char
         col[x] = (int64_t)min(16 * x + c, w - 1) * lda;
         okc[x] = 16 * x + c < w;
     }
-    for (int b = 0; b < nblk; ++b) {
-        const int row0 = b * kRows + wv * 16 + 4 * (lane >> 4);
-        double a[CT][4];
+    // Blocks are read in groups of kGB, the next group's loads in flight
+    // while the current group's products run.
+    constexpr int kGB = 4;
+    float an[kGB][CT][4];
+    auto load_group = [&](int b0) {
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            const int row = row0 + p, rc = min(row, m - 1);
+        for (int q = 0; q < kGB; ++q) {
+            const int row0 = (b0 + q) * kRows + wv * 16 + 4 * (lane >> 4);
 #pragma unroll
-            for (int x = 0; x < CT; ++x) {
-                const float v = P[rc + col[x]];
-                a[x][p] = (row < m && okc[x]) ? (double)v : 0.0;
+            for (int p = 0; p < 4; ++p) {
+                const int row = row0 + p, rc = min(row, m - 1);
+#pragma unroll
+                for (int x = 0; x < CT; ++x) an[q][x][p] = G[rc + col[x]];
             }
         }
-        gram_step<W>(g, a);
+    };
+    load_group(0);
+    for (int b0 = 0; b0 < nblk; b0 += kGB) {
+        float ac[kGB][CT][4];
+#pragma unroll
+        for (int q = 0; q < kGB; ++q)
+#pragma unroll
+            for (int x = 0; x < CT; ++x)
+#pragma unroll
+                for (int p = 0; p < 4; ++p) ac[q][x][p] = an[q][x][p];
+        if (b0 + kGB < nblk) load_group(b0 + kGB);
+#pragma unroll
+        for (int q = 0; q < kGB; ++q) {
+            const int row0 = (b0 + q) * kRows + wv * 16 + 4 * (lane >> 4);
+            double a[CT][4];
+#pragma unroll
+            for (int x = 0; x < CT; ++x)
+#pragma unroll
+                for (int p = 0; p < 4; ++p) a[x][p] = (row0 + p < m && okc[x]) ? (double)ac[q][x][p] : 0.0;
+            gram_step<W>(g, a);
+        }
     }
     gram_finish<W>(g, Cb, part);
     __syncthreads();
@@ -184,21 +217,45 @@ __device__ __noinline__ bool wg_cholqr(int m, int w, float *P, int64_t lda, char
 #pragma unroll
         for (int s2 = 0; s2 < KS; ++s2)
             rv[ct][s2] = (s2 < 4 * (ct + 1)) ? Ris[(4 * s2 + (lane >> 4)) * W + 16 * ct + (lane & 15)] : 0.0;
-    for (int b = 0; b < nblk; ++b) {
-        const int r0 = b * kRows;
-        float av[kTT][W / 4];
-        load_block_mfma<W>(P, lda, m, w, r0, av);
-        const int row = r0 + wv * 16 + (lane & 15);
+    // Groups of kGB blocks with the next group's loads in flight, as above
+    // (the loads read rows the stores of the current group do not touch).
+    // Lane l of wave v reads row 16 v + l % 16 of the block and columns
+    // 4 s2 + l / 16 (the MFMA B layout); addresses clamped, zeros at use.
+    float avn[kGB][W / 4];
+    auto load_apply = [&](int b0) {
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-            doublex4 d = doublex4{0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < kGB; ++q) {
+            const int rc = min((b0 + q) * kRows + wv * 16 + (lane & 15), m - 1);
 #pragma unroll
-            for (int s2 = 0; s2 < 4 * (ct + 1); ++s2)
-                d = __builtin_amdgcn_mfma_f64_16x16x4f64(rv[ct][s2], (double)av[0][s2], d, 0, 0, 0);
+            for (int s2 = 0; s2 < W / 4; ++s2) avn[q][s2] = G[rc + (int64_t)min(4 * s2 + (lane >> 4), w - 1) * lda];
+        }
+    };
+    load_apply(0);
+    for (int b0 = 0; b0 < nblk; b0 += kGB) {
+        float avc[kGB][W / 4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int cl = 16 * ct + (lane >> 4) + 4 * r;
-                if (row < m && cl < w) P[row + (int64_t)cl * lda] = (float)d[r];
+        for (int q = 0; q < kGB; ++q)
+#pragma unroll
+            for (int s2 = 0; s2 < W / 4; ++s2) avc[q][s2] = avn[q][s2];
+        if (b0 + kGB < nblk) load_apply(b0 + kGB);
+#pragma unroll
+        for (int q = 0; q < kGB; ++q) {
+            const int r0 = (b0 + q) * kRows;
+            const int row = r0 + wv * 16 + (lane & 15);
+            float av[W / 4];
+#pragma unroll
+            for (int s2 = 0; s2 < W / 4; ++s2) av[s2] = (row < m && 4 * s2 + (lane >> 4) < w) ? avc[q][s2] : 0.0f;
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                doublex4 d = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s2 = 0; s2 < 4 * (ct + 1); ++s2)
+                    d = __builtin_amdgcn_mfma_f64_16x16x4f64(rv[ct][s2], (double)av[s2], d, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int cl = 16 * ct + (lane >> 4) + 4 * r;
+                    if (row < m && cl < w) G[row + (int64_t)cl * lda] = (float)d[r];
+                }
             }
         }
     }
@@ -207,38 +264,64 @@ __device__ __noinline__ bool wg_cholqr(int m, int w, float *P, int64_t lda, char
 }
 
 // S' (w x nb) = P^T T over all m rows, P the panel (w <= 32 columns), T a
-// block of nb <= NTB earlier columns; wave v holds columns 32 v .. 32 v + 31
-// of S' (MFMA result layout: column l % 32, rows 8 (r / 4) + 4 (l / 32) + r % 4).
+// block of nb <= NTB earlier columns; wave v holds columns 32 (v % (NTB / 32))
+// .. + 31 of S' (MFMA result layout: column l % 32, rows 8 (r / 4) + 4 (l / 32)
+// + r % 4).  With NTB = 64 the waves 2 and 3 take the second k half of every
+// chunk and their sums are added through LDS (Xs) at the end.  Two chunks of
+// loads are in flight (register sets pc0/tc0 and pc1/tc1 alternate): one
+// workgroup streams a whole matrix alone, so its rate is the bytes it keeps in
+// flight over the memory latency.
 template <int PREC, int NTB>
 __device__ __noinline__ floatx16 wg_project(int m, int w, const float *P, const float *T, int64_t lda, int nb, float *Pc,
-                               float *Tc)
+                               float *Tc, float *Xs)
 {
     typedef WgMfma<PREC> M;
+    constexpr int KW = (NTB == 64) ? kKC / 2 : kKC;  // k per wave and chunk
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wc = (NTB == 64) ? (wv & 1) : wv, kq = (NTB == 64) ? (wv >> 1) * KW : 0;
     floatx16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    Chunk<32> pc;
-    Chunk<NTB> tc;
-    pc.load(P, lda, m, 0, w);
-    tc.load(T, lda, m, 0, nb);
-    for (int r0 = 0; r0 < m; r0 += kKC) {
+    Chunk<32> pc0, pc1;
+    Chunk<NTB> tc0, tc1;
+    pc0.load(P, lda, m, 0, w);
+    tc0.load(T, lda, m, 0, nb);
+    if (kKC < m) {
+        pc1.load(P, lda, m, kKC, w);
+        tc1.load(T, lda, m, kKC, nb);
+    }
+    auto step = [&](Chunk<32> &pc, Chunk<NTB> &tc, int r0) {
         __syncthreads();  // the previous chunk has been read
         pc.store(Pc);
         tc.store(Tc);
         __syncthreads();
-        if (r0 + kKC < m) {
-            pc.load(P, lda, m, r0 + kKC, w);
-            tc.load(T, lda, m, r0 + kKC, nb);
+        if (r0 + 2 * kKC < m) {
+            pc.load(P, lda, m, r0 + 2 * kKC, w);
+            tc.load(T, lda, m, r0 + 2 * kKC, nb);
         }
-        if (32 * wv < NTB) {
+        if (32 * wc < NTB) {
 #pragma unroll 4
-            for (int k0 = 0; k0 < kKC; k0 += M::KSTEP) {
+            for (int k0 = kq; k0 < kq + KW; k0 += M::KSTEP) {
                 // A = P^T: A[i][k] = Pc[i][k]; B = T: B[k][j] = Tc[j][k]
                 const auto a = M::frag(Pc, 1, kLDC, lane & 31, k0, lane);
-                const auto b = M::frag(Tc, 1, kLDC, 32 * wv + (lane & 31), k0, lane);
+                const auto b = M::frag(Tc, 1, kLDC, 32 * wc + (lane & 31), k0, lane);
                 acc = M::mma(a, b, acc);
             }
+        }
+    };
+    for (int r0 = 0; r0 < m; r0 += 2 * kKC) {
+        step(pc0, tc0, r0);
+        if (r0 + kKC < m) step(pc1, tc1, r0 + kKC);
+    }
+    if constexpr (NTB == 64) {
+        if (wv >= 2) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) Xs[((wv - 2) * 16 + r) * 64 + lane] = acc[r];
+        }
+        __syncthreads();
+        if (wv < 2) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] += Xs[(wv * 16 + r) * 64 + lane];
         }
     }
     return acc;
@@ -258,27 +341,43 @@ __device__ __noinline__ void wg_update_left(int m, int w, float *P, const float 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int h = wv & 1, kh = wv >> 1;
     constexpr int KH = NTB / 2;
-    Chunk<NTB> tc;
-    tc.load(T, lda, m, 0, nb);
-    float pv[16];
-    auto load_p = [&](int r0) {
-        const int row = r0 + 32 * h + (lane & 31);
+    // Two chunks in flight, as in wg_project: (tc0, pv0) and (tc1, pv1).
+    Chunk<NTB> tc0, tc1;
+    float pv0[16], pv1[16];
+    // P values at clamped addresses; the zeros outside the matrix (and for
+    // the second k half, which adds into the first) are applied at use.
+    auto *G = gmem(P);
+    auto load_p = [&](float (&pv)[16], int r0) {
+        const int rc = min(r0 + 32 * h + (lane & 31), m - 1);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int y = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            pv[r] = (kh == 0 && row < m && y < w) ? P[row + (int64_t)y * lda] : 0.0f;
+            pv[r] = G[rc + (int64_t)min(y, w - 1) * lda];
         }
     };
-    load_p(0);
-    for (int r0 = 0; r0 < m; r0 += kKC) {
+    tc0.load(T, lda, m, 0, nb);
+    load_p(pv0, 0);
+    if (kKC < m) {
+        tc1.load(T, lda, m, kKC, nb);
+        load_p(pv1, kKC);
+    }
+    auto step = [&](Chunk<NTB> &tc, float (&pv)[16], int r0) {
         __syncthreads();
         tc.store(Tc);
         __syncthreads();
-        if (r0 + kKC < m) tc.load(T, lda, m, r0 + kKC, nb);
         floatx16 acc;
+        {
+            const int row = r0 + 32 * h + (lane & 31);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = pv[r];
-        if (r0 + kKC < m) load_p(r0 + kKC);
+            for (int r = 0; r < 16; ++r) {
+                const int y = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                acc[r] = (kh == 0 && row < m && y < w) ? pv[r] : 0.0f;
+            }
+        }
+        if (r0 + 2 * kKC < m) {
+            tc.load(T, lda, m, r0 + 2 * kKC, nb);
+            load_p(pv, r0 + 2 * kKC);
+        }
         const int x0 = 32 * h;
 #pragma unroll 4
         for (int k0 = kh * KH; k0 < kh * KH + KH; k0 += M::KSTEP) {
@@ -297,9 +396,13 @@ __device__ __noinline__ void wg_update_left(int m, int w, float *P, const float 
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int y = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (row < m && y < w) P[row + (int64_t)y * lda] = acc[r] + Xs[(h * 16 + r) * 64 + lane];
+                if (row < m && y < w) G[row + (int64_t)y * lda] = acc[r] + Xs[(h * 16 + r) * 64 + lane];
             }
         }
+    };
+    for (int r0 = 0; r0 < m; r0 += 2 * kKC) {
+        step(tc0, pv0, r0);
+        if (r0 + kKC < m) step(tc1, pv1, r0 + kKC);
     }
 }
 
@@ -318,7 +421,7 @@ __device__ void wg_factor(const WgTask &tk, char *lds, bool reorth)
     // R's strictly-lower triangle is zero (the blocks above are all written).
     for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += kThreads) {
         const int i = (int)(e % n), j = (int)(e / n);
-        if (i > j) R[i + (int64_t)j * ldr] = 0.0f;
+        if (i > j) gmem(R)[i + (int64_t)j * ldr] = 0.0f;
     }
     bool bad = false;
     for (int c0 = 0; c0 < n; c0 += BW) {
@@ -330,7 +433,7 @@ __device__ void wg_factor(const WgTask &tk, char *lds, bool reorth)
             for (int q0 = 0; q0 < c0; q0 += NTB) {
                 const int nb = min(NTB, c0 - q0);
                 const float *T = A + (int64_t)q0 * lda;
-                const floatx16 acc = wg_project<PREC, NTB>(m, w, P, T, lda, nb, Pc, Tc);
+                const floatx16 acc = wg_project<PREC, NTB>(m, w, P, T, lda, nb, Pc, Tc, Xs);
                 __syncthreads();  // Tc and Pc have been read
                 if (32 * wv < NTB) {
                     const int j = 32 * wv + (lane & 31);  // column of the block
@@ -339,7 +442,7 @@ __device__ void wg_factor(const WgTask &tk, char *lds, bool reorth)
                         const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);  // column of the panel
                         St[j * kLDT + i] = -acc[r];
                         if (i < w && j < nb) {
-                            float *d = R + q0 + j + (int64_t)(c0 + i) * ldr;
+                            auto *d = gmem(R) + q0 + j + (int64_t)(c0 + i) * ldr;
                             *d = pass ? *d + acc[r] : acc[r];
                         }
                     }
@@ -364,7 +467,7 @@ __device__ void wg_factor(const WgTask &tk, char *lds, bool reorth)
                 const int i = e % w, j = e / w;
                 double s = 0.0;
                 for (int k = i; k <= j; ++k) s = fma(Rs[i * 32 + k], R1[k * 32 + j], s);
-                R[c0 + i + (int64_t)(c0 + j) * ldr] = (i <= j) ? (float)s : 0.0f;
+                gmem(R)[c0 + i + (int64_t)(c0 + j) * ldr] = (i <= j) ? (float)s : 0.0f;
             }
             __syncthreads();
         }

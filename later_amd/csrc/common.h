@@ -79,4 +79,17 @@ __device__ __forceinline__ double wave_sum(double v)
     return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
+// Global-address-space view of a device pointer.  A pointer the compiler
+// cannot trace to a kernel argument (one read from a task table, or a
+// parameter of a function that is not inlined) is generic, and its loads and
+// stores become flat instructions; those count in lgkmcnt as well as vmcnt,
+// so every wait for an LDS access also waits for them, and loads issued
+// ahead of their use stop overlapping anything.  Accesses through this view
+// are global loads and stores.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 __host__ __device__ static inline int64_t ceil_div64(int64_t a, int64_t b) { return (a + b - 1) / b; }
