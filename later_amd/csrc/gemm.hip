@@ -105,6 +105,9 @@ template <int R, bool KM, int BK> struct Slab {
     __device__ __forceinline__ void load(const float *P, int64_t ld, int r0, int rmax, int k0, int kmax,
                                          bool vec)
     {
+        // Global view: the grouped kernels read P from an argument table.
+        const auto *G = gmem(P);
+        typedef const __attribute__((address_space(1))) floatx4 GV4;
         const int t = threadIdx.x;
         if (KM) {
             const int k = k0 + 4 * (t % CPL);
@@ -112,9 +115,9 @@ template <int R, bool KM, int BK> struct Slab {
             for (int p = 0; p < NPASS; ++p) {
                 const int row = r0 + p * RPI + t / CPL;
                 const bool rin = row < rmax;
-                const float *q = P + (int64_t)min(row, rmax - 1) * ld;
+                const auto *q = G + (int64_t)min(row, rmax - 1) * ld;
                 if (vec && rin && k + 4 <= kmax) {
-                    floatx4 x = *(const floatx4 *)(q + k);
+                    floatx4 x = *(GV4 *)(q + k);
                     v[4 * p] = x[0]; v[4 * p + 1] = x[1]; v[4 * p + 2] = x[2]; v[4 * p + 3] = x[3];
                 } else {
 #pragma unroll
@@ -130,7 +133,7 @@ template <int R, bool KM, int BK> struct Slab {
             if (vec && rg + 4 <= rmax && kb + KG <= kmax) {
 #pragma unroll
                 for (int kk = 0; kk < KG; ++kk) {
-                    floatx4 x = *(const floatx4 *)(P + (int64_t)(kb + kk) * ld + rg);
+                    floatx4 x = *(GV4 *)(G + (int64_t)(kb + kk) * ld + rg);
 #pragma unroll
                     for (int rr = 0; rr < 4; ++rr) v[kk * 4 + rr] = x[rr];
                 }
@@ -139,7 +142,7 @@ template <int R, bool KM, int BK> struct Slab {
                 for (int kk = 0; kk < KG; ++kk)
 #pragma unroll
                     for (int rr = 0; rr < 4; ++rr)
-                        v[kk * 4 + rr] = P[(int64_t)min(kb + kk, kmax - 1) * ld + min(rg + rr, rmax - 1)];
+                        v[kk * 4 + rr] = G[(int64_t)min(kb + kk, kmax - 1) * ld + min(rg + rr, rmax - 1)];
 #pragma unroll
                 for (int kk = 0; kk < KG; ++kk)
 #pragma unroll
@@ -242,6 +245,7 @@ template <int R, int BK, int NP> struct Slab16 {
     __device__ __forceinline__ void load(const uint16_t *P, int64_t ld, int64_t piece, int r0, int rmax, int k0,
                                          int kmax, bool vec)
     {
+        const auto *G = gmem(P);
         const int t = threadIdx.x;
         const int k = k0 + 8 * (t % CPL);
 #pragma unroll
@@ -250,9 +254,9 @@ template <int R, int BK, int NP> struct Slab16 {
             const bool rin = row < rmax;
 #pragma unroll
             for (int c = 0; c < NP; ++c) {
-                const uint16_t *q = P + c * piece + (int64_t)min(row, rmax - 1) * ld;
+                const auto *q = G + c * piece + (int64_t)min(row, rmax - 1) * ld;
                 if (vec && rin && k + 8 <= kmax) {
-                    v[c][p] = *(const ushort8 *)(q + k);
+                    v[c][p] = *(const __attribute__((address_space(1))) ushort8 *)(q + k);
                 } else {
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
@@ -331,7 +335,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs &g, int split, int row
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const int col = col0 + 32 * j + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                    cv[j][r] = (col < g.N) ? g.C[(int64_t)col * g.ldc + row] : 0.0f;
+                    cv[j][r] = (col < g.N) ? gmem(g.C)[(int64_t)col * g.ldc + row] : 0.0f;
                 }
         }
 #pragma unroll
@@ -340,18 +344,18 @@ __device__ __forceinline__ void store_tile(const GemmArgs &g, int split, int row
             for (int r = 0; r < 16; ++r) {
                 const int col = col0 + 32 * j + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (col >= g.N) continue;
-                const float cs = oscale_a * ((g.b_exp) ? ldexpf(1.0f, -g.b_exp[col]) : 1.0f);
+                const float cs = oscale_a * ((g.b_exp) ? ldexpf(1.0f, -gmem(g.b_exp)[col]) : 1.0f);
                 const double v = value(i, j, r) * (double)cs;
                 if (g.partial) {
                     const int64_t off = ((int64_t)split * g.N + col) * g.M + row;
-                    if constexpr (CH64) ((double *)g.partial)[off] = v;
-                    else ((float *)g.partial)[off] = (float)v;
+                    if constexpr (CH64) gmem((double *)g.partial)[off] = v;
+                    else gmem((float *)g.partial)[off] = (float)v;
                 } else if (g.update) {
-                    float *c = g.C + (int64_t)col * g.ldc + row;
+                    auto *c = gmem(g.C) + (int64_t)col * g.ldc + row;
                     if (CH64 || g.update == 2) *c = (float)((double)cv[j][r] + (double)g.alpha * v);
                     else *c = fmaf(g.alpha, (float)v, cv[j][r]);
                 } else {
-                    g.C[(int64_t)col * g.ldc + row] = (float)v;
+                    gmem(g.C)[(int64_t)col * g.ldc + row] = (float)v;
                 }
             }
         }

@@ -34,7 +34,7 @@ __device__ __forceinline__ void mgs_block_body(int m, int w, float *A, int64_t l
     const int mm = (b == nblk - 1) ? (m - r0) : bh;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    float *Ab = A + r0;
+    auto *Ab = gmem(A) + r0;  // global view (the grouped kernel reads A from a table)
 
     // Loads from clamped in-bounds addresses, selected afterwards, so that all
     // of them issue before the first wait.
@@ -107,7 +107,7 @@ __device__ __forceinline__ void mgs_block_body(int m, int w, float *A, int64_t l
         }
     }
     // R block (w x w, zeros below the diagonal) at rows b*w of Rout.
-    float *Rb = Rout + (int64_t)b * w;
+    auto *Rb = gmem(Rout) + (int64_t)b * w;
     for (int i = threadIdx.x; i < w * w; i += kThreads) {
         const int x = i % w, y = i / w;
         Rb[x + (int64_t)y * ldr] = (x <= y) ? Rs[x][y] : 0.0f;
@@ -140,13 +140,13 @@ __device__ __forceinline__ void tsqr_apply_body(int m, int w, float *A, int64_t 
     __shared__ float Ws[kCols][kCols];
     const int r0 = b * bh;
     const int mm = (b == nblk - 1) ? (m - r0) : bh;
-    const float *Wb = W + (int64_t)b * w;
+    const auto *Wb = gmem(W) + (int64_t)b * w;
     for (int i = threadIdx.x; i < w * w; i += 256) {
         const int x = i % w, y = i / w;
         Ws[x][y] = Wb[x + (int64_t)y * ldw];
     }
     __syncthreads();
-    float *Ab = A + r0;
+    auto *Ab = gmem(A) + r0;
     for (int row = threadIdx.x; row < mm; row += 256) {
         // Unconditional loads (clamped column) so all of them are in flight at once.
         float v[kCols];
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void tsqr_apply_tile_kernel(int m, int w, floa
     const int row0 = blockIdx.x * kApplyRows;
     const int b = min(row0 / bh, nblk - 1);
     const int rows = min(kApplyRows, m - row0);
-    const float *Wb = W + (int64_t)b * w;
+    const auto *Wb = gmem(W) + (int64_t)b * w;
     for (int i = threadIdx.x; i < w * w; i += 256) {
         const int x = i % w, y = i / w;
         Ws[x][y] = Wb[x + (int64_t)y * ldw];
