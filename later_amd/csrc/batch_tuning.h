@@ -5,7 +5,7 @@
 // columns m classes (m < 128, < 256, ..., >= 4096).
 #pragma once
 constexpr int kDefaultTuning[kBuckets] = {
-    1, 1, 1, 1, 1, 3, 1,
+    3, 1, 3, 1, 1, 1, 3,
     1, 1, 1, 1, 1, 1, 1,
     1, 1, 1, 1, 1, 1, 1,
     1, 1, 1, 1, 1, 1, 1,
