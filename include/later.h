@@ -66,12 +66,19 @@ enum {
                              products, power-of-two column scaling */
     LATER_GEMM_F16X6 = 4, /* FP16 three-way split (2^-33 representation), six
                              MFMA products; projections only (updates use X3) */
-    LATER_GEMM_FP8 = 5    /* FP8 e4m3 operands, each operand column scaled by a
+    LATER_GEMM_FP8 = 5,   /* FP8 e4m3 operands, each operand column scaled by a
                              power of two per 128 rows of K (the E8M0 block
                              scales of v_mfma_scale_f32_32x32x64_f8f6f4), FP32
                              accumulation; projections only (updates default
                              to X3); the operands are staged in hwork, which
                              this precision requires */
+    LATER_GEMM_I8X4 = 6   /* int8 four-slice split of both operands (each
+                             column scaled by a power of two, x 2^e =
+                             s0 + s1 2^-7 + s2 2^-14 + s3 2^-21 with |s| <= 64),
+                             the thirteen products of slice pairs i + j <= 4
+                             on v_mfma_i32_32x32x32_i8, summed exactly in
+                             int32; projections only, K <= 131008 rows
+                             (longer products use X6), staged in hwork */
 };
 
 /* Panel (leaf) algorithm. */
@@ -116,8 +123,10 @@ typedef struct later_opts {
  * below, FP32 block-MGS TSQR panel, no re-orthogonalisation. */
 void later_opts_default(later_opts *o);
 /* The accuracy-recovering mixed-precision form: FP16X3 first pass (FP32
- * accumulation, chunk 0) above 256 columns, FP16X6 second-pass projections
- * above 128 columns (FP64 folds every 32 rows), FP32 products below,
+ * accumulation, chunk 0) above 256 columns, I8X4 second-pass projections
+ * above 128 columns (FP16X6 with FP64 folds every 32 rows when hwork cannot
+ * stage the slices or m > 131008), FP16X3 second-pass updates above 256
+ * columns, FP32 products below,
  * Cholesky QR leaves of up to 64 columns (one pass per leaf,
  * LATER_PANEL_CHOLQR1), second full pass. */
 void later_opts_mixed(later_opts *o);

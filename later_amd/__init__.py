@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "liblater_amd.so")
 
 # Status codes (include/later.h)
 OK, ERR_SHAPE, ERR_WORKSPACE, ERR_OPTION, ERR_DEVICE, ERR_BREAKDOWN, ERR_ALIAS, ERR_BUDGET, ERR_CANCELLED = range(9)
-GEMM_F32, GEMM_F16, GEMM_BF16, GEMM_F16X3, GEMM_F16X6, GEMM_FP8 = range(6)
+GEMM_F32, GEMM_F16, GEMM_BF16, GEMM_F16X3, GEMM_F16X6, GEMM_FP8, GEMM_I8X4 = range(7)
 PANEL_MGS, PANEL_MGS64, PANEL_CHOLQR2, PANEL_CHOLQR1 = range(4)
 REORTH_NONE, REORTH_NODE, REORTH_PASS = range(3)
 

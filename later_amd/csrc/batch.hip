@@ -363,6 +363,11 @@ LATER_API int later_batch_plan_create(int count, const int *m, const int *n, flo
     later_opts o;
     if (opts) o = *opts; else later_opts_default(&o);
     if (o.reorth & LATER_REORTH_NODE) return LATER_ERR_OPTION;  // not offered for batches
+    // The grouped kernels convert their operands in the kernel; int8 slices
+    // need a staged, column-scaled operand, so those nodes run the
+    // three-piece FP16 split (as later_rgsqrf_ex does without its arena).
+    if (o.gemm == LATER_GEMM_I8X4) o.gemm = LATER_GEMM_F16X6;
+    if (o.gemm_pass2 == LATER_GEMM_I8X4) o.gemm_pass2 = LATER_GEMM_F16X6;
     // The same option checks as later_rgsqrf_ex (a chunk below 32 would fold
     // the FP64 sums only after the last k).
     auto prec_ok = [](int p) { return p >= LATER_GEMM_F32 && p <= LATER_GEMM_F16X6; };

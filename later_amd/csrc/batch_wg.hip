@@ -554,7 +554,7 @@ LATER_API int later_batch_wg_create(int count, const int *m, const int *n, float
     later_opts o;
     if (opts) o = *opts; else later_opts_default(&o);
     if (count < 1 || !m || !n || !A || !lda || !R || !ldr) return LATER_ERR_SHAPE;
-    if (o.gemm < LATER_GEMM_F32 || o.gemm > LATER_GEMM_FP8 || o.reorth < 0 || o.reorth > 3) return LATER_ERR_OPTION;
+    if (o.gemm < LATER_GEMM_F32 || o.gemm > LATER_GEMM_I8X4 || o.reorth < 0 || o.reorth > 3) return LATER_ERR_OPTION;
     std::vector<WgTask> tasks(count);
     std::vector<std::pair<double, int>> order(count);
     for (int i = 0; i < count; ++i) {

@@ -3,7 +3,8 @@
 #include "common.h"
 
 enum { GEMM_F32 = LATER_GEMM_F32, GEMM_F16 = LATER_GEMM_F16, GEMM_BF16 = LATER_GEMM_BF16,
-       GEMM_F16X3 = LATER_GEMM_F16X3, GEMM_F16X6 = LATER_GEMM_F16X6, GEMM_FP8 = LATER_GEMM_FP8 };
+       GEMM_F16X3 = LATER_GEMM_F16X3, GEMM_F16X6 = LATER_GEMM_F16X6, GEMM_FP8 = LATER_GEMM_FP8,
+       GEMM_I8X4 = LATER_GEMM_I8X4 };
 
 // One product of the tiled MFMA kernel (device-side argument block; the
 // grouped launches read an array of these).
@@ -28,6 +29,10 @@ struct GemmArgs {
     // FP8 operands (GEMM_FP8): Ah / Bh hold bytes (ld in bytes), the E8M0
     // block scales are at byte offset psa / psb with row stride lsa / lsb
     // (scale of row i, K block kb at kb * ls + i).
+    // int8 slices (GEMM_I8X4): K, kper, ldah / ldbh and psa / psb count
+    // pairs of bytes (so the 16-bit streaming code serves them unchanged);
+    // the int32 column exponents of op(A)'s rows and of B's columns are at
+    // byte offset lsa / lsb from Ah / Bh.
     int64_t lsa, lsb;
 };
 
@@ -35,7 +40,7 @@ struct GemmArgs {
 struct Pieces {
     const uint16_t *p;
     int64_t ld, stride;
-    int64_t ls;  // FP8 only: row stride of the block scales
+    int64_t ls;  // FP8: row stride of the block scales; I8X4: byte offset of the exponents
 };
 
 // FP8 staging (GEMM_FP8) of an operand whose K runs down its columns: column
@@ -50,6 +55,23 @@ int64_t fp8_scale_off(int K, int cols);
 int64_t fp8_bytes(int K, int cols);  // whole staging buffer, a multiple of 256
 Pieces fp8_pieces(const void *buf, int K, int cols);
 int gemm_split8(hipStream_t s, int K, int cols, const float *X, int64_t ldx, void *buf, int *launches);
+
+// int8 staging (GEMM_I8X4) of an operand whose K runs down its columns:
+// column j of X gets the exponent e_j with max |X(:,j)| 2^e_j in [32, 64),
+// and slice c of its scaled values (rows padded with zeros to a multiple of
+// 64) sits at buf + c * i8_stride(K, cols) + j * i8_ld(K), bytes; the int32
+// e_j at buf + i8_exp_off(K, cols) + 4 j.
+constexpr int kI8MaxK = 131008;  // exact int32 sums: 4 x 64 x 64 x K < 2^31 (2^17 - 64)
+int64_t i8_ld(int K);
+int64_t i8_stride(int K, int cols);
+int64_t i8_exp_off(int K, int cols);
+int64_t i8_bytes(int K, int cols);  // whole staging buffer, a multiple of 256
+Pieces i8_pieces(const void *buf, int K, int cols);
+int gemm_split_i8(hipStream_t s, int K, int cols, const float *X, int64_t ldx, void *buf, int *launches);
+// The same slices into an explicit layout: column j at out + j * ld, slice c
+// at + c * stride, its exponent at exps[j] (a column range of a wider array).
+int gemm_split_i8_into(hipStream_t s, int K, int cols, const float *X, int64_t ldx, void *out, int64_t ld,
+                       int64_t stride, int *exps, int *launches);
 
 struct ColexpArgs {
     int m;

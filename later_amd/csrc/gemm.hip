@@ -57,12 +57,16 @@ template <> struct Lds<GEMM_F16> { typedef _Float16 T; };
 template <> struct Lds<GEMM_BF16> { typedef __bf16 T; };
 template <> struct Lds<GEMM_F16X3> { typedef _Float16 T; };
 template <> struct Lds<GEMM_F16X6> { typedef _Float16 T; };
+template <> struct Lds<GEMM_I8X4> { typedef int8_t T; };
 
 // FP16 pieces per operand value: x = p0 + p1 2^-11 + p2 2^-22, with
 // p0 = fl16(x), p1 = fl16((x - p0) 2^11), p2 = fl16((x - p0 - p1 2^-11) 2^22).
 // Products are summed per power of 2^-11 in separate FP32 accumulators:
 // group g holds sum_{i+j=g} p_i q_j.
-template <int PREC> constexpr int pieces() { return PREC == GEMM_F16X6 ? 3 : PREC == GEMM_F16X3 ? 2 : 1; }
+template <int PREC> constexpr int pieces()
+{
+    return PREC == GEMM_I8X4 ? 4 : PREC == GEMM_F16X6 ? 3 : PREC == GEMM_F16X3 ? 2 : 1;
+}
 
 // The MFMA input pieces of one (already scaled) FP32 value.  The fused
 // conversion of the slab stores and the split kernels both use this, so
@@ -287,6 +291,7 @@ template <> struct Frag<GEMM_F16> { typedef half8_t T; };
 template <> struct Frag<GEMM_BF16> { typedef bf16x8_t T; };
 template <> struct Frag<GEMM_F16X3> { typedef half8_t T; };
 template <> struct Frag<GEMM_F16X6> { typedef half8_t T; };
+template <> struct Frag<GEMM_I8X4> { typedef intx4 T; };  // 16 int8 k per lane
 
 template <int PREC>
 __device__ __forceinline__ floatx16 mfma(typename Frag<PREC>::T a, typename Frag<PREC>::T b, floatx16 c)
@@ -314,11 +319,12 @@ __device__ __forceinline__ typename Frag<PREC>::T frag(const typename Lds<PREC>:
 // Epilogue of one wave's FM x FN accumulator blocks (value(i, j, r), FP64,
 // still scaled).  Transposed 32x32 layout: lane l holds row l%32 of the block
 // and columns 8 (r/4) + 4 (l/32) + r%4, r = 0..15.
-template <bool CH64, int FM, int FN, typename V>
+// ROWEXP: rows of op(A) carry their own exponents (int8 slices, at byte
+// offset lsa from Ah); the result is scaled by 2^-(a_exp + row's + column's).
+template <bool CH64, int FM, int FN, typename V, bool ROWEXP = false>
 __device__ __forceinline__ void store_tile(const GemmArgs &g, int split, int row0, int col0, V value)
 {
     const int lane = threadIdx.x & 63;
-    const float oscale_a = ldexpf(1.0f, -g.a_exp);
     const bool rmw = !g.partial && g.update;
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
@@ -328,6 +334,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs &g, int split, int row
         // flight together (interleaved with the stores they would wait one
         // memory latency each: the compiler cannot move a load of C above a
         // store to C).
+        const int rexp = ROWEXP ? gmem((const int *)((const char *)g.Ah + g.lsa))[row] : 0;
         float cv[FN][16];
         if (rmw) {
 #pragma unroll
@@ -344,8 +351,8 @@ __device__ __forceinline__ void store_tile(const GemmArgs &g, int split, int row
             for (int r = 0; r < 16; ++r) {
                 const int col = col0 + 32 * j + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (col >= g.N) continue;
-                const float cs = oscale_a * ((g.b_exp) ? ldexpf(1.0f, -gmem(g.b_exp)[col]) : 1.0f);
-                const double v = value(i, j, r) * (double)cs;
+                const int esum = g.a_exp + rexp + ((g.b_exp) ? gmem(g.b_exp)[col] : 0);
+                const double v = value(i, j, r) * ldexp(1.0, -esum);
                 if (g.partial) {
                     const int64_t off = ((int64_t)split * g.N + col) * g.M + row;
                     if constexpr (CH64) gmem((double *)g.partial)[off] = v;
@@ -635,7 +642,13 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
     constexpr int BK = 32, WM = BM / WGM, WN = 64, FM = WM / 32, FN = 2, KSTEPS = 2;
     constexpr int BPA = BM / 16 / NW, BPB = BN / 16 / NW;  // 1 KB blocks per wave per piece tile
     static_assert(BPA >= 1 && BPB >= 1 && FM >= 1, "tile shape");
-    constexpr int NP = pieces<PREC>(), NG = NP;
+    constexpr bool I8 = PREC == GEMM_I8X4;  // int8 slices: 64 k per slab, exact int32 sums
+    // Accumulator groups: slice pairs of weight 2^-11g (FP16 pieces, g below
+    // the piece count) or 2^-7g (int8 slices, g <= 4: the pairs of weight
+    // 2^-28 are as large as a column's typical product times 2^-21, so they
+    // stay; the first pairs dropped weigh 2^-35).
+    constexpr int NP = pieces<PREC>(), NG = I8 ? 5 : NP;
+    static_assert(!(I8 && CH64), "int8 products need no FP64 folds");
     constexpr int PBA = BM * BK * 2, PBB = BN * BK * 2;  // bytes of one piece of one operand tile
     constexpr int SB = NP * (PBA + PBB);                 // bytes of one stage
     constexpr int NS = (163840 / SB) < 4 ? (163840 / SB) : 4;
@@ -688,7 +701,8 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
         return *(const FT *)(pbase + row * 64 + ch * 16);
     };
 
-    floatx16 acc[NG][FM][FN];
+    typedef typename std::conditional<I8, intx16, floatx16>::type AccT;
+    AccT acc[NG][FM][FN];
     double acc64[CH64 ? FM : 1][CH64 ? FN : 1][CH64 ? 16 : 1];
 #pragma unroll
     for (int q = 0; q < NG; ++q)
@@ -697,7 +711,7 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
 #pragma unroll
             for (int j = 0; j < FN; ++j)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) acc[q][i][j][r] = 0.0f;
+                for (int r = 0; r < 16; ++r) acc[q][i][j][r] = 0;
     if constexpr (CH64) {
 #pragma unroll
         for (int i = 0; i < FM; ++i)
@@ -707,10 +721,18 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
                 for (int r = 0; r < 16; ++r) acc64[i][j][r] = 0.0;
     }
     auto combined = [&](int i, int j, int r) -> double {
-        double v = (double)acc[0][i][j][r];
-        if constexpr (NG > 1) v += (double)acc[1][i][j][r] * 0x1p-11;
-        if constexpr (NG > 2) v += (double)acc[2][i][j][r] * 0x1p-22;
-        return v;
+        if constexpr (I8) {
+            // Slice-pair sums of weight 2^-7g; each is exact, the FP64 sum
+            // rounds once per term.
+            return ((((double)acc[0][i][j][r] + (double)acc[1][i][j][r] * 0x1p-7) +
+                     (double)acc[2][i][j][r] * 0x1p-14) + (double)acc[3][i][j][r] * 0x1p-21) +
+                   (double)acc[4][i][j][r] * 0x1p-28;
+        } else {
+            double v = (double)acc[0][i][j][r];
+            if constexpr (NG > 1) v += (double)acc[1][i][j][r] * 0x1p-11;
+            if constexpr (NG > 2) v += (double)acc[2][i][j][r] * 0x1p-22;
+            return v;
+        }
     };
     auto fold_lead = [&]() {
         if constexpr (CH64) {
@@ -777,12 +799,17 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
 #pragma unroll
             for (int gq = 0; gq < NG; ++gq)
 #pragma unroll
-                for (int a = 0; a <= gq; ++a)
+                for (int a = (gq < NP ? 0 : gq - NP + 1); a <= (gq < NP ? gq : NP - 1); ++a)
 #pragma unroll
                     for (int i = 0; i < FM; ++i)
 #pragma unroll
-                        for (int j = 0; j < FN; ++j)
-                            acc[gq][i][j] = mfma<PREC>(fb[ks][gq - a][j], fa[ks][a][i], acc[gq][i][j]);
+                        for (int j = 0; j < FN; ++j) {
+                            if constexpr (I8)
+                                acc[gq][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[ks][gq - a][j], fa[ks][a][i],
+                                                                                      acc[gq][i][j], 0, 0, 0);
+                            else
+                                acc[gq][i][j] = mfma<PREC>(fb[ks][gq - a][j], fa[ks][a][i], acc[gq][i][j]);
+                        }
         }
         // FP64 folds (kLeadFolds), counted rather than divided.
         if constexpr (CH64) {
@@ -798,10 +825,11 @@ __device__ __forceinline__ void gemm_lds_body(const GemmArgs &g, const int tm, c
         }
     }
 
-    store_tile<CH64, FM, FN>(g, split, m0 + wm * WM, n0 + wn * WN, [&](int i, int j, int r) -> double {
+    auto value = [&](int i, int j, int r) -> double {
         if constexpr (CH64) return acc64[i][j][r];
         else return combined(i, j, r);
-    });
+    };
+    store_tile<CH64, FM, FN, decltype(value), I8>(g, split, m0 + wm * WM, n0 + wn * WN, value);
 }
 
 // Tile order of a triangular product (one split).  Tile row tm has work
@@ -995,6 +1023,100 @@ __global__ __launch_bounds__(256) void split8_kernel(int K, const float *X, int6
     w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * s, v[3] * s, w, true);
     if (k < prow) *(int *)(out + (int64_t)j * ldo + k) = w;
     if ((threadIdx.x & 31) == 0 && k < K) sc[(int64_t)(k >> 7) * ls + j] = (uint8_t)(127 - e);
+}
+
+// FP32 -> int8 slices (see i8_ld), one workgroup per column: the column's
+// largest magnitude mx = f 2^E (f in [0.5, 1)) gives e = 6 - E, so |x 2^e| < 64;
+// then s0 = rint(x 2^e) and s_c = rint(128 r_{c-1}) with r_c the remainder
+// after slice c (each remainder is exact in FP32 and |r| <= 1/2, so every
+// slice lies in [-64, 64]).  Rows K .. roundup(K, 64) - 1 are zeros.
+// Thread t owns rows 4 t + 1024 i .. + 3: columns of up to kI8RegRows rows
+// are read once into registers (16-byte loads when the column is aligned),
+// longer ones are read twice (the maximum, then the slices).
+constexpr int kI8RegRows = 16384;
+
+__device__ __forceinline__ void i8_words(floatx4 v, float sc, int (&w)[4])
+{
+#pragma unroll
+    for (int c = 0; c < 4; ++c) w[c] = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float r = v[q] * sc;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float sl = rintf(r);
+            r = (r - sl) * 128.0f;
+            w[c] |= ((int)sl & 0xff) << (8 * q);
+        }
+    }
+}
+
+__device__ __forceinline__ floatx4 i8_load4(const __attribute__((address_space(1))) float *x, int k, int K, bool vec)
+{
+    if (vec && k + 4 <= K) return *(const __attribute__((address_space(1))) floatx4 *)(x + k);
+    floatx4 v;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = (k + q < K) ? x[k + q] : 0.0f;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void split_i8_kernel(int K, const float *X, int64_t ldx, int8_t *out, int64_t ld,
+                                                       int64_t stride, int *exps)
+{
+    __shared__ float red[4];
+    const int j = blockIdx.x, t = threadIdx.x;
+    const auto *x = gmem(X + (int64_t)j * ldx);
+    const bool vec = (((uintptr_t)(X + (int64_t)j * ldx)) & 15) == 0;
+    const int prow = (K + 63) / 64 * 64;
+    constexpr int NI = kI8RegRows / 1024;
+    floatx4 v[NI];
+    float mx = 0.0f;
+    const bool inreg = prow <= kI8RegRows;
+    if (inreg) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int k = 4 * t + 1024 * i;
+            v[i] = (k < K) ? i8_load4(x, k, K, vec) : floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[i][0]), fabsf(v[i][1])), fmaxf(fabsf(v[i][2]), fabsf(v[i][3]))));
+    } else {
+        for (int k = 4 * t; k < K; k += 1024) {
+            const floatx4 a = i8_load4(x, k, K, vec);
+            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(a[0]), fabsf(a[1])), fmaxf(fabsf(a[2]), fabsf(a[3]))));
+        }
+    }
+#pragma unroll
+    for (int mask = 1; mask < 64; mask <<= 1) mx = fmaxf(mx, __shfl_xor(mx, mask, 64));
+    if ((t & 63) == 0) red[t >> 6] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    int e = 0;
+    if (mx > 0.0f && mx < INFINITY) {
+        int fe;
+        frexpf(mx, &fe);
+        e = 6 - fe;
+        e = e < -126 ? -126 : (e > 126 ? 126 : e);
+    }
+    if (t == 0) gmem(exps)[j] = e;
+    const float sc = ldexpf(1.0f, e);
+    auto *o = gmem(out + (int64_t)j * ld);
+    auto put = [&](int k, floatx4 a) {
+        int w[4];
+        i8_words(a, sc, w);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) *(__attribute__((address_space(1))) int *)(o + c * stride + k) = w[c];
+    };
+    if (inreg) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int k = 4 * t + 1024 * i;
+            if (k < prow) put(k, v[i]);
+        }
+    } else {
+        for (int k = 4 * t; k < prow; k += 1024) put(k, (k < K) ? i8_load4(x, k, K, vec) : floatx4{0.0f, 0.0f, 0.0f, 0.0f});
+    }
 }
 
 // Grouped launch: workgroup b computes tile (t.y, t.z) of split t.w of the
@@ -1329,6 +1451,14 @@ int dispatch(hipStream_t s, int prec, const GemmArgs &g, int nsplit, bool small)
         }
         break;
     }
+    case GEMM_I8X4: {
+        // Staged int8 slices only, 128 x 128 tiles on eight waves (four
+        // int32 accumulator groups per 32 x 64 wave tile), every shape.
+        if (!pre || !AKM || CH64) return LATER_ERR_WORKSPACE;
+        const dim3 grid((unsigned)ceil_div64(g.M, 128), (unsigned)ceil_div64(g.N, 128), (unsigned)nsplit);
+        hipLaunchKernelGGL((gemm_lds_kernel<GEMM_I8X4, false, 128, 4>), grid, dim3(512), 0, s, g);
+        break;
+    }
     default: return LATER_ERR_OPTION;
     }
     HIP_OK(hipGetLastError());
@@ -1436,12 +1566,14 @@ int gemm_proj(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, in
               int64_t ldb, float *C, int64_t ldc, int chunk, const int *b_exp, int a_exp, float *work,
               int64_t wfloats, int *launches, const Pieces *qh, const Pieces *bh)
 {
-    if (prec == GEMM_FP8) chunk = 0;  // exact e4m3 products, FP32 sums: no FP64 folds
+    const bool i8 = prec == GEMM_I8X4;
+    if (prec == GEMM_FP8 || i8) chunk = 0;  // exact e4m3 / int8 products: no FP64 folds
     const bool ch64 = chunk > 0;
     if (ch64 && (chunk % 32 || (chunk < kMaxBK && kMaxBK % chunk))) return LATER_ERR_OPTION;
     bool small;
     int ns, kp;
     proj_plan(m, n1, n2, chunk, &small, &ns, &kp);
+    if (i8 && (kp > kI8MaxK || !qh || !bh)) return kp > kI8MaxK ? LATER_ERR_SHAPE : LATER_ERR_WORKSPACE;
     if (ns > 1 && wfloats < (int64_t)ns * n1 * n2 * (ch64 ? 2 : 1)) return LATER_ERR_WORKSPACE;
     GemmArgs g;
     g.M = n1; g.N = n2; g.K = m;
@@ -1454,6 +1586,15 @@ int gemm_proj(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, in
     g.alpha = 1.0f;
     g.tri = 0;
     set_pieces(g, qh, bh);
+    if (i8) {
+        // Byte operands counted in pairs of bytes (K padded to 64 rows);
+        // the exponents come with the staged slices.
+        g.K = (m + 63) / 64 * 32;
+        g.kper = kp / 2;
+        g.ldah /= 2; g.psa /= 2; g.ldbh /= 2; g.psb /= 2;
+        g.a_exp = 0;
+        g.b_exp = (const int *)((const char *)bh->p + bh->ls);
+    }
     const int tok = prof_start(s);
     int rc = ch64 ? dispatch<true, true>(s, prec, g, ns, small) : dispatch<true, false>(s, prec, g, ns, small);
     // Unfolded hi/lo projections on pre-split pieces run the updates' 256-row
@@ -1626,4 +1767,38 @@ int gemm_split8(hipStream_t s, int K, int cols, const float *X, int64_t ldx, voi
     HIP_OK(hipGetLastError());
     if (launches) ++*launches;
     return LATER_OK;
+}
+
+int64_t i8_ld(int K) { return ((int64_t)K + 63) / 64 * 64 + 64; }
+int64_t i8_stride(int K, int cols) { return (i8_ld(K) * cols + 255) / 256 * 256; }
+int64_t i8_exp_off(int K, int cols) { return 4 * i8_stride(K, cols); }
+int64_t i8_bytes(int K, int cols) { return i8_exp_off(K, cols) + (4 * (int64_t)cols + 255) / 256 * 256; }
+
+Pieces i8_pieces(const void *buf, int K, int cols)
+{
+    Pieces p;
+    p.p = (const uint16_t *)buf;
+    p.ld = i8_ld(K);
+    p.stride = i8_stride(K, cols);
+    p.ls = i8_exp_off(K, cols);
+    return p;
+}
+
+int gemm_split_i8_into(hipStream_t s, int K, int cols, const float *X, int64_t ldx, void *out, int64_t ld,
+                       int64_t stride, int *exps, int *launches)
+{
+    if (K <= 0 || cols <= 0) return LATER_OK;
+    if (ld % 64 || ld < (K + 63) / 64 * 64 || stride % 64) return LATER_ERR_SHAPE;
+    hipLaunchKernelGGL(split_i8_kernel, dim3((unsigned)cols), dim3(256), 0, s, K, X, ldx, (int8_t *)out, ld, stride,
+                       exps);
+    HIP_OK(hipGetLastError());
+    if (launches) ++*launches;
+    return LATER_OK;
+}
+
+int gemm_split_i8(hipStream_t s, int K, int cols, const float *X, int64_t ldx, void *buf, int *launches)
+{
+    int8_t *b = (int8_t *)buf;
+    return gemm_split_i8_into(s, K, cols, X, ldx, b, i8_ld(K), i8_stride(K, cols), (int *)(b + i8_exp_off(K, cols)),
+                              launches);
 }

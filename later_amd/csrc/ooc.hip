@@ -52,9 +52,14 @@ int split_cols(int n) { return 32 * ((n + 63) / 64); }
 int upd_prec(const later_opts &o, int p)
 {
     if (o.gemm_update >= 0) return o.gemm_update;
-    return (p == LATER_GEMM_F16X6 || p == LATER_GEMM_FP8) ? LATER_GEMM_F16X3 : p;
+    return (p == LATER_GEMM_F16X6 || p == LATER_GEMM_FP8 || p == LATER_GEMM_I8X4) ? LATER_GEMM_F16X3 : p;
 }
-int node_prec(int p) { return p == LATER_GEMM_FP8 ? LATER_GEMM_F16X3 : p; }
+// The sweeps' block products convert in the kernel: int8 slices (which need
+// a staged, column-scaled operand) run as the three-piece FP16 split.
+int node_prec(int p)
+{
+    return p == LATER_GEMM_FP8 ? LATER_GEMM_F16X3 : (p == LATER_GEMM_I8X4 ? LATER_GEMM_F16X6 : p);
+}
 
 // FP64 running sum of one stream's block projections.
 __global__ void acc_kernel(int n1, int n2, const float *C, double *acc, int first)
@@ -342,8 +347,8 @@ LATER_API int later_oc_qr_rec(laterCtxt ctxt, int m, int n, float *A, int lda, f
     c.bytes = device_bytes / kAlign * kAlign;
     int rc = LATER_OK;
     if (m < 1 || n < 1 || m < n || lda < m || ldr < n || !A || !R) rc = LATER_ERR_SHAPE;
-    else if (c.o.gemm < LATER_GEMM_F32 || c.o.gemm > LATER_GEMM_FP8 || c.o.gemm_pass2 < LATER_GEMM_F32 ||
-             c.o.gemm_pass2 > LATER_GEMM_FP8 || c.o.gemm_update < -1 || c.o.gemm_update > LATER_GEMM_F16X3 ||
+    else if (c.o.gemm < LATER_GEMM_F32 || c.o.gemm > LATER_GEMM_I8X4 || c.o.gemm_pass2 < LATER_GEMM_F32 ||
+             c.o.gemm_pass2 > LATER_GEMM_I8X4 || c.o.gemm_update < -1 || c.o.gemm_update > LATER_GEMM_F16X3 ||
              c.o.panel < 0 || c.o.panel > 3 || c.o.reorth < 0 || c.o.reorth > 3 || c.o.nmin < 0 || c.o.nmin2 < 0 ||
              c.o.chunk < 0 || c.o.chunk % 32 || c.o.chunk2 < 0 || c.o.chunk2 % 32)
         rc = LATER_ERR_OPTION;

@@ -31,11 +31,25 @@ constexpr int kQExp = 14;          // |Q| <= 1: Q scaled by 2^14 before 16-bit c
 
 int split_n(int n) { return 32 * ((n + 63) / 64); }
 
-bool prec_ok(int p) { return p >= LATER_GEMM_F32 && p <= LATER_GEMM_FP8; }
+bool prec_ok(int p) { return p >= LATER_GEMM_F32 && p <= LATER_GEMM_I8X4; }
 bool chol_panel(int p) { return p == LATER_PANEL_CHOLQR2 || p == LATER_PANEL_CHOLQR1; }
 bool scaled(int p) { return p == LATER_GEMM_F16X3 || p == LATER_GEMM_F16X6; }
-// F16X6 and FP8 are projection precisions; their updates use the hi/lo pair.
-int update_prec(int p) { return (p == LATER_GEMM_F16X6 || p == LATER_GEMM_FP8) ? LATER_GEMM_F16X3 : p; }
+// F16X6, FP8 and I8X4 are projection precisions; their updates use the hi/lo pair.
+int update_prec(int p)
+{
+    return (p == LATER_GEMM_F16X6 || p == LATER_GEMM_FP8 || p == LATER_GEMM_I8X4) ? LATER_GEMM_F16X3 : p;
+}
+// Byte-staged projection operands (each node splits its Q1 and A2 itself).
+bool staged8(int p) { return p == LATER_GEMM_FP8 || p == LATER_GEMM_I8X4; }
+int64_t staged_bytes(int p, int m, int n) { return p == LATER_GEMM_FP8 ? fp8_bytes(m, n) : i8_bytes(m, n); }
+// The projection precision a node of height m runs: int8 slices need their
+// staging arena (and no block CGS2 at the nodes, whose re-projection
+// converts in the kernel) and at most kI8MaxK rows (exact int32 sums);
+// otherwise the three-piece FP16 split, the next most accurate form.
+int proj_eff(int p, int m, bool staged)
+{
+    return (p == LATER_GEMM_I8X4 && (!staged || m > kI8MaxK)) ? LATER_GEMM_F16X6 : p;
+}
 // The update precision of the nodes whose projections run `p`.
 int update_of(const later_opts &o, int p) { return o.gemm_update >= 0 ? o.gemm_update : update_prec(p); }
 bool update_ok(int p) { return p == -1 || (p >= LATER_GEMM_F32 && p <= LATER_GEMM_F16X3); }
@@ -76,7 +90,8 @@ bool pass_pieces(int n, int nmin_p, int p, int pu)
 }
 
 // 16-bit elements the per-node staged operands of one node need: the e4m3
-// bytes of Q1 and A2 for FP8 projections (any width above nmin); for nodes
+// bytes of Q1 and A2 for FP8 projections, the int8 slices of A2 for I8X4
+// (whose Q1 slices persist like Q's pieces), at any width above nmin; for nodes
 // of kPieceMinN columns and more, A2 as 16-bit pieces for the projection and
 // R12 for the update (Q's pieces persist across the nodes of a pass).
 int64_t node_scratch(int m, int n, int p, int pu)
@@ -85,8 +100,9 @@ int64_t node_scratch(int m, int n, int p, int pu)
     const int64_t ldm = piece_ld(m), ldn1 = piece_ld(n1);
     int64_t tot = 0;
     if (p == LATER_GEMM_FP8) tot += (fp8_bytes(m, n1) + fp8_bytes(m, n2)) / 2;
+    if (p == LATER_GEMM_I8X4) tot += i8_bytes(m, n2) / 2;
     if (n >= kPieceMinN && gemm_pieces(pu) > 0) {
-        if (p != LATER_GEMM_FP8) tot += proj_pieces(p) * ldm * n2;
+        if (!staged8(p)) tot += proj_pieces(p) * ldm * n2;
         tot += gemm_pieces(pu) * ldn1 * n2;
     }
     return tot;
@@ -124,12 +140,14 @@ Layout layout(int m, int n, const later_opts &o)
     // the largest node's own staged operands (the top node's).
     int64_t hcol = 0, hrow = 0, hnode = 0;
     for (int pass = 0; pass < ((o.reorth & LATER_REORTH_PASS) ? 2 : 1); ++pass) {
-        const int p = pass ? o.gemm_pass2 : o.gemm, pu = update_of(o, p);
+        const int p = proj_eff(pass ? o.gemm_pass2 : o.gemm, m, !(o.reorth & LATER_REORTH_NODE)),
+                  pu = update_of(o, p);
         const int nmin_p = pass_nmin_p(o, pass == 1);
         if (p == LATER_GEMM_F32 || n <= nmin_p) continue;
         hnode = std::max(hnode, node_scratch(m, n, p, pu));
+        if (p == LATER_GEMM_I8X4) hcol = std::max(hcol, i8_bytes(m, n) / 2);  // Q's slices, any width
         if (pass_pieces(n, nmin_p, p, pu)) {
-            if (p != LATER_GEMM_FP8) hcol = std::max(hcol, proj_pieces(p) * piece_ld(m) * n);
+            if (!staged8(p)) hcol = std::max(hcol, proj_pieces(p) * piece_ld(m) * n);
             hrow = std::max(hrow, gemm_pieces(pu) * piece_ld(n) * m);
         }
     }
@@ -158,6 +176,7 @@ struct Run {
     uint16_t *qrow;    // Q's pieces, transposed (ld ldqr, piece stride qrow_ps)
     uint16_t *npcs;    // the current node's own staged operands
     int64_t ldqc, ldqr, qcol_ps, qrow_ps;
+    int64_t q8ld, q8ps, q8exp;  // int8 passes: Q's slices at qcol (bytes: ld, slice stride, exponents)
     std::vector<unsigned char> col_done, row_done;  // columns of Q split in this pass
     int launches;
     int depth;
@@ -226,7 +245,10 @@ int ensure_q(Run &r, int prec, int m, int c0, int cols, const float *A, int64_t 
         int b = a;
         while (b < c0 + cols && !done[b]) done[b++] = 1;
         const float *X = A + (int64_t)(a - c0) * lda;
-        if (ROW)
+        if (!ROW && prec == LATER_GEMM_I8X4)
+            rc = gemm_split_i8_into(r.s, m, b - a, X, lda, (char *)r.qcol + a * r.q8ld, r.q8ld, r.q8ps,
+                                    (int *)((char *)r.qcol + r.q8exp) + a, &r.launches);
+        else if (ROW)
             rc = gemm_split_t_exact(r.s, prec, m, b - a, X, lda, aexp, r.qrow + a, r.ldqr, r.qrow_ps, &r.launches);
         else
             rc = gemm_split(r.s, prec, m, b - a, X, lda, aexp, nullptr, r.qcol + (int64_t)a * r.ldqc, r.ldqc,
@@ -250,7 +272,8 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     }
     const int n1 = split_n(n), n2 = n - n1;
     const bool inner = (n <= r.nmin_p), inner_u = (n <= r.nmin_u);
-    const int p = inner ? LATER_GEMM_F32 : prec;                     // projection precision
+    // projection precision
+    const int p = inner ? LATER_GEMM_F32 : proj_eff(prec, m, r.pcs && !(r.o.reorth & LATER_REORTH_NODE));
     const int pu = inner_u ? LATER_GEMM_F32 : update_of(r.o, prec);  // update precision
     const int chunk = inner ? 0 : r.chunk;
     const bool f64 = r.chunk > 0 && !inner_u;  // split updates subtract in FP64 when the pass folds
@@ -264,18 +287,27 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     // as FP16 pieces): the same MFMA inputs as the fused conversion, made
     // once per node instead of once per tile.  FP8 projections are always
     // staged (their block scales need a pass over each 128-row block).
-    const bool fp8 = p == LATER_GEMM_FP8;
+    const bool fp8 = staged8(p);  // FP8 or int8: staged per node, always
     const bool pre = r.pcs && !inner_u && n >= kPieceMinN && gemm_pieces(pu) > 0;
     const bool pre_proj = fp8 ? (r.pcs && !inner) : (pre && proj_pieces(p) > 0);
     if (fp8 && !pre_proj) return LATER_ERR_WORKSPACE;
     Pieces q1h{}, a2h{}, q1t{}, r12h{};
     {
         uint16_t *b = r.npcs;
-        if (fp8) {
+        if (p == LATER_GEMM_FP8) {
             q1h = fp8_pieces(b, m, n1);
             b += fp8_bytes(m, n1) / 2;
             a2h = fp8_pieces(b, m, n2);
             b += fp8_bytes(m, n2) / 2;
+        } else if (p == LATER_GEMM_I8X4) {
+            // Q1: the pass's persistent slices from column c0 (its exponents
+            // at a fixed offset from the array base); A2: this node's own.
+            q1h.p = (const uint16_t *)((const char *)r.qcol + c0 * r.q8ld);
+            q1h.ld = r.q8ld;
+            q1h.stride = r.q8ps;
+            q1h.ls = r.q8exp + 4 * (int64_t)c0 - c0 * r.q8ld;
+            a2h = i8_pieces(b, m, n2);
+            b += i8_bytes(m, n2) / 2;
         } else if (pre_proj) {
             const int64_t ldm = piece_ld(m);
             q1h = {r.qcol + (int64_t)c0 * r.ldqc, r.ldqc, r.qcol_ps};
@@ -289,9 +321,12 @@ int qr_rec(Run &r, int m, int n, float *A, int64_t lda, float *R, int64_t ldr, i
     }
     // R12 = Q1^T A2   (QR/later_rgsqrf.cu:43-49); A2's column exponents and
     // pieces in one launch when it is staged.
-    if (fp8) {
+    if (p == LATER_GEMM_FP8) {
         if ((rc = gemm_split8(r.s, m, n1, A, lda, (void *)q1h.p, &r.launches))) return rc;
         if ((rc = gemm_split8(r.s, m, n2, A2, lda, (void *)a2h.p, &r.launches))) return rc;
+    } else if (p == LATER_GEMM_I8X4) {
+        if ((rc = ensure_q<false>(r, p, m, c0, n1, A, lda, 0))) return rc;
+        if ((rc = gemm_split_i8(r.s, m, n2, A2, lda, (void *)a2h.p, &r.launches))) return rc;
     } else {
         if (pre_proj && (rc = ensure_q<false>(r, p, m, c0, n1, A, lda, aexp))) return rc;
         if (pre_proj && scaled(p)) {
@@ -379,23 +414,27 @@ LATER_API void later_opts_fp32(later_opts *o)
 // Mixed form: FP16 hi/lo products accumulated in FP32 over the whole depth
 // in the first pass (its projections only have to leave a Q that the second
 // pass can repair: FP64 folds every 256 rows change the final orthogonality
-// from 8.66e-6 to 8.60e-6 at 16384^2 and cost 2.5 ms); three-piece products
-// folded every 32 rows for the second pass's projections, which set the
-// orthogonality floor; FP32 products at nodes of up to 256 columns (128 for
+// from 8.66e-6 to 8.60e-6 at 16384^2 and cost 2.5 ms); int8 four-slice
+// products summed exactly in int32 for the second pass's projections, which
+// set the orthogonality floor (the three-piece FP16 form with FP64 folds
+// every 32 rows where the slices cannot be staged or a node is taller than
+// 131008 rows); FP32 products at nodes of up to 256 columns (128 for
 // the second pass's projections), which are memory-bound: the FP32 MFMA
 // runs them faster than the split forms and rounds less (262144 x 256:
 // 2.18 vs 2.85 ms); Cholesky QR leaves of up to 64 columns, one pass each
 // (the second factorisation repairs what a second Cholesky QR pass of a
 // first-pass leaf would).  At 16384^2 (N(0,1)) the form gives
-// ||I - Q^T Q||_F ~ 8.7e-6 and ||A - QR||_F / ||A||_F = 3.4e-7 (graded
-// singular values 1 .. 1e-4: 9.6e-6 and 5.8e-7); single FP16 first-pass
+// ||I - Q^T Q||_F ~ 6.6e-6 and ||A - QR||_F / ||A||_F = 3.4e-7 (graded
+// singular values 1 .. 1e-4: 7.6e-6 and 5.8e-7); with three-piece
+// second-pass projections 8.7e-6 (graded 9.6e-6); single FP16 first-pass
 // projections give 1.02e-5 (graded 4.1e-5), two-piece second-pass
-// projections 1.3e-5, single FP16 second-pass updates 2.2e-4.
+// projections 1.3e-5, int8 slices without the weight-2^-28 pairs 1.4e-5,
+// single FP16 second-pass updates 2.2e-4.
 LATER_API void later_opts_mixed(later_opts *o)
 {
     later_opts_default(o);
     o->gemm = LATER_GEMM_F16X3;
-    o->gemm_pass2 = LATER_GEMM_F16X6;
+    o->gemm_pass2 = LATER_GEMM_I8X4;
     o->panel = LATER_PANEL_CHOLQR1;
     o->reorth = LATER_REORTH_PASS;
     o->nmin = 256;
@@ -494,6 +533,9 @@ LATER_API int later_rgsqrf_ex(laterCtxt ctxt, int m, int n, float *A, int lda, f
     r.ldqr = piece_ld(n);
     r.qcol_ps = r.ldqc * n;
     r.qrow_ps = r.ldqr * m;
+    r.q8ld = i8_ld(m);
+    r.q8ps = i8_stride(m, n);
+    r.q8exp = i8_exp_off(m, n);
     r.col_done.assign(n, 0);
     r.row_done.assign(n, 0);
     r.launches = 0;
@@ -619,7 +661,7 @@ LATER_API int later_caqr_panel(laterCtxt ctxt, int m, int w, float *A, int lda, 
 // piece_ld(n1), m rows) and C column-major (ld piece_ld(n1)).
 static int64_t component_pieces(int m, int n1, int n2, int prec, bool update)
 {
-    if (prec == LATER_GEMM_FP8) return update ? 0 : (fp8_bytes(m, n1) + fp8_bytes(m, n2)) / 2;
+    if (staged8(prec)) return update ? 0 : (staged_bytes(prec, m, n1) + staged_bytes(prec, m, n2)) / 2;
     const int np = gemm_pieces(prec);
     if (np == 0) return 0;
     if (!update) return (int64_t)np * piece_ld(m) * ((int64_t)n1 + n2);
@@ -630,8 +672,8 @@ LATER_API int later_gemm_workspace(int m, int n1, int n2, int prec, int chunk, i
                                    int64_t *lhwork)
 {
     if (m < 1 || n1 < 1 || n2 < 1) return LATER_ERR_SHAPE;
-    if (!prec_ok(prec) || chunk < 0 || chunk % 32 || (update && prec == LATER_GEMM_FP8)) return LATER_ERR_OPTION;
-    if (prec == LATER_GEMM_FP8) chunk = 0;
+    if (!prec_ok(prec) || chunk < 0 || chunk % 32 || (update && staged8(prec))) return LATER_ERR_OPTION;
+    if (staged8(prec)) chunk = 0;
     const int64_t exps = ((int64_t)n2 + 63) / 64 * 64;
     if (lwork) *lwork = update ? exps : exps + gemm_proj_work_floats(m, n1, n2, chunk);
     if (lhwork) *lhwork = component_pieces(m, n1, n2, prec, update != 0);
@@ -644,7 +686,7 @@ LATER_API int later_gemm_proj(laterCtxt ctxt, int m, int n1, int n2, const float
 {
     if (m < 1 || n1 < 1 || n2 < 1 || ldq < m || ldb < m || ldc < n1 || !Q || !B || !C) return LATER_ERR_SHAPE;
     if (!prec_ok(prec) || chunk < 0 || chunk % 32) return LATER_ERR_OPTION;
-    const bool fp8 = prec == LATER_GEMM_FP8;
+    const bool fp8 = staged8(prec);
     if (fp8) chunk = 0;
     const int64_t exps = ((int64_t)n2 + 63) / 64 * 64;
     const int64_t need = exps + gemm_proj_work_floats(m, n1, n2, chunk);
@@ -653,11 +695,18 @@ LATER_API int later_gemm_proj(laterCtxt ctxt, int m, int n1, int n2, const float
     const bool pre = hwork && hneed > 0 && lhwork >= hneed && (((uintptr_t)hwork) & 255) == 0;
     hipStream_t s = ctx_stream(ctxt);
     if (fp8) {
-        // e4m3 operands with per-128-row block scales, staged in hwork.
+        // e4m3 operands with per-128-row block scales, or int8 slices with
+        // column exponents, staged in hwork.
         if (!pre) return LATER_ERR_WORKSPACE;
-        const Pieces qh = fp8_pieces(hwork, m, n1), bh = fp8_pieces(hwork + fp8_bytes(m, n1) / 2, m, n2);
-        int rc = gemm_split8(s, m, n1, Q, ldq, (void *)qh.p, nullptr);
-        if (!rc) rc = gemm_split8(s, m, n2, B, ldb, (void *)bh.p, nullptr);
+        const bool i8 = prec == LATER_GEMM_I8X4;
+        if (i8 && m > kI8MaxK) return LATER_ERR_SHAPE;
+        const Pieces qh = i8 ? i8_pieces(hwork, m, n1) : fp8_pieces(hwork, m, n1);
+        const uint16_t *hb = hwork + staged_bytes(prec, m, n1) / 2;
+        const Pieces bh = i8 ? i8_pieces(hb, m, n2) : fp8_pieces(hb, m, n2);
+        int rc = i8 ? gemm_split_i8(s, m, n1, Q, ldq, (void *)qh.p, nullptr)
+                    : gemm_split8(s, m, n1, Q, ldq, (void *)qh.p, nullptr);
+        if (!rc) rc = i8 ? gemm_split_i8(s, m, n2, B, ldb, (void *)bh.p, nullptr)
+                         : gemm_split8(s, m, n2, B, ldb, (void *)bh.p, nullptr);
         if (rc) return rc;
         return gemm_proj(s, prec, m, n1, n2, Q, ldq, B, ldb, C, ldc, 0, nullptr, 0, work + exps, lwork - exps,
                          nullptr, &qh, &bh);
@@ -688,7 +737,7 @@ LATER_API int later_gemm_update(laterCtxt ctxt, int m, int n1, int n2, const flo
                                 int64_t lhwork)
 {
     if (m < 1 || n1 < 1 || n2 < 1 || ldq < m || ldb < m || ldc < n1 || !Q || !B || !C) return LATER_ERR_SHAPE;
-    if (!prec_ok(prec) || prec == LATER_GEMM_FP8) return LATER_ERR_OPTION;
+    if (!prec_ok(prec) || staged8(prec)) return LATER_ERR_OPTION;
     const int64_t exps = ((int64_t)n2 + 63) / 64 * 64;
     if (scaled(prec) && (lwork < exps || !work)) return LATER_ERR_WORKSPACE;
     const int64_t hneed = component_pieces(m, n1, n2, prec, true);

@@ -403,6 +403,10 @@ LATER_API int later_server_create(int slots, int64_t hbm_budget_bytes, const lat
         S->qr_opts.chunk = 256;
     }
     later_opts_mixed(&S->qr_robust);
+    // Block CGS2 at every node re-projects with operands converted in the
+    // kernel, so the second pass's projections use the three-piece FP16
+    // split rather than staged int8 slices.
+    S->qr_robust.gemm_pass2 = LATER_GEMM_F16X6;
     S->qr_robust.reorth = LATER_REORTH_NODE | LATER_REORTH_PASS;
     S->qr_robust.chunk = 256;  // ill-conditioned requests: FP64 folds in both passes
     S->pool_bytes = hbm_budget_bytes / 256 * 256;

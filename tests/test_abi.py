@@ -34,7 +34,7 @@ def test_options(later):
     assert o.gemm == later.GEMM_F32
     o = later.opts("mixed")
     assert o.gemm == later.GEMM_F16X3 and o.reorth == later.REORTH_PASS
-    assert o.gemm_pass2 == later.GEMM_F16X6 and o.chunk == 0 and o.chunk2 == 32
+    assert o.gemm_pass2 == later.GEMM_I8X4 and o.chunk == 0 and o.chunk2 == 32
     assert o.nmin == 256 and o.nmin2 == 128
     assert later.opts("reference").nmin2 == 0
 

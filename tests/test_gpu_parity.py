@@ -242,15 +242,17 @@ def test_rgsqrf_deterministic(later, oracle):
         assert np.array_equal(Q1, Q2) and np.array_equal(R1, R2), form
 
 
-@pytest.mark.parametrize("form,m,n", [("reference", 4096, 1024), ("mixed", 3000, 1100), ("fp32", 2048, 1024),
-                                     ("mixed", 2048, 2048)])
+@pytest.mark.parametrize("form,m,n", [("reference", 4096, 1024), ("mixed6", 3000, 1100), ("fp32", 2048, 1024),
+                                     ("mixed6", 2048, 2048)])
 def test_presplit_pieces_give_fused_bits(later, oracle, form, m, n):
     """Nodes of 512+ columns stage their 16-bit operands as pre-split pieces
     in the 16-bit arena `hwork` when it is as large as the query asks; one
     element short of it (or without it), the GEMMs convert in the kernel.
-    Both give the same bits."""
+    Both give the same bits.  (mixed6: the mixed form with three-piece
+    second-pass projections; its int8 slices exist only staged, see
+    test_int8_needs_the_arena.)"""
     A = oracle.gen_uniform(m, n, 0, 0)
-    o = later.opts(form)
+    o = later.opts("mixed", gemm_pass2=later.GEMM_F16X6) if form == "mixed6" else later.opts(form)
     lw, lh = later.workspace_sizes(m, n, o)
     if form == "fp32":  # FP32 products stage nothing
         assert lh == 0
@@ -329,6 +331,30 @@ def test_workspace_too_small_is_an_error(later):
     Rf = later.rgsqrf(A0, o=later.opts("mixed"), hwork=None)
     torch.cuda.synchronize()
     assert torch.equal(R, Rf) and torch.equal(A, A0)
+
+
+def test_int8_needs_the_arena(later, oracle):
+    """The mixed form's int8 second-pass projections need their slices
+    staged in hwork: one element short of the queried arena (or without
+    it) those nodes run the three-piece FP16 split, whose bits the run then
+    has, and both meet the gates."""
+    import torch
+    m, n = 2048, 2048
+    A = oracle.gen_uniform(m, n, 0, 0)
+    o8, o6 = later.opts("mixed"), later.opts("mixed", gemm_pass2=later.GEMM_F16X6)
+    hw = later.hworkspace(m, n, o8)
+    out = []
+    for o, h in ((o8, hw), (o8, hw[:-1]), (o8, None), (o6, None)):
+        Ad = _dev(later, A)
+        R = later.rgsqrf(Ad, o=o, hwork=h)
+        _sync()
+        out.append((Ad, R))
+    assert not torch.equal(out[0][1], out[1][1])
+    for q in out[2:]:
+        assert torch.equal(out[1][0], q[0]) and torch.equal(out[1][1], q[1])
+    A0 = _dev(later, A)
+    for Q, R in out[:2]:
+        assert later.check_orth(Q) < 1e-5 and later.check_backward(A0, Q, R) < 1e-6
 
 
 # ---------------------------------------------------------------- -check
