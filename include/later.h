@@ -353,11 +353,13 @@ int later_check_backward(laterCtxt ctxt, int m, int n, const float *A, int lda, 
                          int ldq, const float *R, int ldr, double *backward);
 
 /* Kernel-class timing with HIP events on the launch stream (instrumentation
- * for benchmarks; off by default).  Classes: 0 node projection GEMM, 1 node
- * update GEMM (other than class 4), 2 panel block-MGS kernel, 3 other, 4 node
- * update GEMM of the FP16 hi/lo form on pre-split operands with 128 x 128
- * tiles (a single kernel).  later_prof_end synchronises and fills 5-entry
- * arrays: launches, total ms, algorithmic flops and algorithmic HBM bytes.
+ * for benchmarks; off by default).  Classes: 0 node projection GEMM (other
+ * than classes 4 and 5), 1 node update GEMM (other than class 4), 2 panel
+ * block-MGS kernel, 3 other, 4 the FP16 hi/lo products on pre-split operands
+ * with 256 x 128 tiles (a single kernel: node updates, the R2 R1
+ * composition, large first-pass projections), 5 the int8 four-slice
+ * projections.  later_prof_end synchronises and fills 6-entry arrays:
+ * launches, total ms, algorithmic flops and algorithmic HBM bytes.
  * Host-side, process-wide state. */
 void later_prof_begin(void);
 int later_prof_end(int *launches, double *ms, double *flops, double *bytes);

@@ -607,7 +607,7 @@ class Server:
             self.h = None
 
 
-PROF_CLASSES = ("gemm_proj", "gemm_update", "panel_mgs", "other", "gemm_update_x3")
+PROF_CLASSES = ("gemm_proj", "gemm_update", "panel_mgs", "other", "gemm_update_x3", "proj_i8")
 
 
 def prof_begin():

@@ -1600,7 +1600,7 @@ int gemm_proj(hipStream_t s, int prec, int m, int n1, int n2, const float *Q, in
     // Unfolded hi/lo projections on pre-split pieces run the updates' 256-row
     // streaming kernel: they are timed in its class.
     const bool x3 = prec == GEMM_F16X3 && !ch64 && !small && g.Ah && g.Bh;
-    prof_stop(tok, s, x3 ? PROF_UPDATE_X3 : PROF_GEMM_PROJ, 2.0 * m * n1 * n2,
+    prof_stop(tok, s, i8 ? PROF_PROJ_I8 : (x3 ? PROF_UPDATE_X3 : PROF_GEMM_PROJ), 2.0 * m * n1 * n2,
               4.0 * ((double)m * (n1 + n2) + (double)n1 * n2));
     if (rc) return rc;
     if (launches) ++*launches;

@@ -7,9 +7,11 @@
 // PROF_UPDATE_X3: every launch of the streaming FP16 hi/lo kernel on 256-row
 // tiles (pre-split operands, no FP64 folds; the mixed form's largest): the
 // node updates, the R2 R1 composition and the first pass's projections of
-// enough tiles; PROF_GEMM_UPDATE and PROF_GEMM_PROJ hold the other products.
+// enough tiles; PROF_PROJ_I8: the int8 four-slice projections (the mixed
+// form's second pass); PROF_GEMM_UPDATE and PROF_GEMM_PROJ hold the other
+// products.
 enum { PROF_GEMM_PROJ = 0, PROF_GEMM_UPDATE = 1, PROF_PANEL = 2, PROF_OTHER = 3, PROF_UPDATE_X3 = 4,
-       PROF_NCLASS = 5 };
+       PROF_PROJ_I8 = 5, PROF_NCLASS = 6 };
 
 // Returns a token for prof_stop, or -1 when profiling is off.
 int prof_start(hipStream_t s);
