@@ -182,7 +182,8 @@ def blas_ceiling(torch, M=16384, N=8192, K=8192, reps=5):
 
 def square_variants(later, torch, A0, steps=2):
     """The 16384^2 square workload's other forms: FP16 and BF16 node GEMMs
-    with and without the second pass (Cholesky QR leaves when repeated), on
+    with and without the second pass (Cholesky QR leaves when repeated), FP32
+    products with the second pass, on
     the same N(0,1) input; and the graded input A = U diag(s) V^T (U, V the
     Q factors of two Gaussian matrices, s geometric from 1 to 1e-4) in the
     mixed and plain-FP32 forms."""
@@ -195,6 +196,9 @@ def square_variants(later, torch, A0, steps=2):
         "bf16": later.opts("reference", gemm=later.GEMM_BF16, gemm_pass2=later.GEMM_BF16),
         "bf16_reorth": later.opts("reference", gemm=later.GEMM_BF16, gemm_pass2=later.GEMM_BF16,
                                   reorth=later.REORTH_PASS, panel=later.PANEL_CHOLQR1),
+        # The plain-FP32 form with the mixed form's second pass and leaves:
+        # FP32 products everywhere (does FP32 reach the gate at all?).
+        "fp32_reorth": later.opts("fp32", reorth=later.REORTH_PASS, panel=later.PANEL_CHOLQR1),
     }
 
     def run(name, A, o):
