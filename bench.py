@@ -234,7 +234,7 @@ def square_variants(later, torch, A0, steps=2):
     return out
 
 
-def batched_config(later, torch, count=4096, steps=3):
+def batched_config(later, torch, count=4096, steps=3, single=False):
     """4096 independent matrices, m log-uniform in [64, 4096], n uniform in
     [16, min(m, 256)], entries 2u - 1 (Philox, seed 0, matrix index i).  The
     headline engine is the persistent kernel (one launch; a workgroup per
@@ -252,6 +252,13 @@ def batched_config(later, torch, count=4096, steps=3):
     flops = float(np.sum(2.0 * m * n * n))
     nbytes = float(np.sum(8.0 * m * n))
     s = torch.cuda.current_stream()
+    if single:  # one persistent launch of the FP32-product form, for profilers
+        plan = later.BatchWg(As, Rs, later.opts("mixed"))
+        plan.execute()
+        torch.cuda.synchronize()
+        plan.close()
+        log(f"[bench] one persistent batched launch over {count} matrices done")
+        return None
 
     def timed(plan):
         times = []
@@ -511,6 +518,8 @@ def main():
                     help="skip the secondary workloads (8192x1024, tall-skinny, batched, serving)")
     ap.add_argument("--single", choices=("mixed", "fp32", "reference"),
                     help="one eager factorisation of the headline workload and nothing else (for profilers)")
+    ap.add_argument("--single-batched", action="store_true",
+                    help="one persistent launch of the 4096-matrix batched workload and nothing else (for profilers)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="three 1 GiB device-to-device copies (the HBM counter calibration)")
     args = ap.parse_args()
@@ -532,6 +541,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         log("[bench] copied 3 x 1 GiB")
+        return
+    if args.single_batched:
+        batched_config(later, torch, single=True)
         return
     if args.single:
         o = later.opts(args.single)
