@@ -1218,19 +1218,11 @@ __global__ void colexp_grouped_kernel(const ColexpArgs *args, const int2 *tasks)
 // thread converts 4 consecutive rows of one column.
 // Rows rows .. roundup(rows, 64) - 1 (the K padding the streaming GEMM
 // reads) are written as zeros.
-// Eight rows i0 .. i0 + 7 of column x into its pieces o + c * stride: two
-// 16-byte loads and one 16-byte store per piece where the column and its
-// piece rows are aligned; rows rows .. prow - 1 are written as zeros.
-template <int PREC>
-__device__ __forceinline__ void split_rows8(const float *x, uint16_t *o, int rows, int prow, int64_t stride, int i0,
-                                            float sc)
+// Rows i0 .. i0 + 7 of column x (zeros past `rows`): two 16-byte loads where
+// the column is aligned.
+__device__ __forceinline__ void load_rows8(const float *x, int rows, int i0, float (&v)[8])
 {
-    typedef typename Lds<PREC>::T T;
-    constexpr int NP = pieces<PREC>();
-    const bool vec = i0 + 8 <= rows && ((((uintptr_t)(x + i0)) & 15) == 0) && ((((uintptr_t)(o + i0)) & 15) == 0) &&
-                     (stride & 7) == 0;
-    float v[8];
-    if (vec) {
+    if (i0 + 8 <= rows && ((((uintptr_t)(x + i0)) & 15) == 0)) {
         const floatx4 a = *(const floatx4 *)(x + i0), b = *(const floatx4 *)(x + i0 + 4);
 #pragma unroll
         for (int e = 0; e < 4; ++e) { v[e] = a[e]; v[4 + e] = b[e]; }
@@ -1238,6 +1230,16 @@ __device__ __forceinline__ void split_rows8(const float *x, uint16_t *o, int row
 #pragma unroll
         for (int e = 0; e < 8; ++e) v[e] = (i0 + e < rows) ? x[i0 + e] : 0.0f;
     }
+}
+
+// Pieces of rows i0 .. i0 + 7 (values v, zeros past `rows`) at o + c * stride.
+template <int PREC>
+__device__ __forceinline__ void store_rows8(const float (&v)[8], uint16_t *o, int rows, int prow, int64_t stride,
+                                            int i0, float sc)
+{
+    typedef typename Lds<PREC>::T T;
+    constexpr int NP = pieces<PREC>();
+    const bool vec = i0 + 8 <= rows && ((((uintptr_t)(o + i0)) & 15) == 0) && (stride & 7) == 0;
     ushort8 pv[NP];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -1256,6 +1258,18 @@ __device__ __forceinline__ void split_rows8(const float *x, uint16_t *o, int row
 #pragma unroll
                 for (int c = 0; c < NP; ++c) o[c * stride + i0 + e] = pv[c][e];
     }
+}
+
+// Eight rows i0 .. i0 + 7 of column x into its pieces o + c * stride: two
+// 16-byte loads and one 16-byte store per piece where the column and its
+// piece rows are aligned; rows rows .. prow - 1 are written as zeros.
+template <int PREC>
+__device__ __forceinline__ void split_rows8(const float *x, uint16_t *o, int rows, int prow, int64_t stride, int i0,
+                                            float sc)
+{
+    float v[8];
+    load_rows8(x, rows, i0, v);
+    store_rows8<PREC>(v, o, rows, prow, stride, i0, sc);
 }
 
 // Pieces of X scaled by 2^e (e = a_exp or col_exp[j]), same layout: each
@@ -1277,12 +1291,54 @@ __global__ void split_kernel(int rows, int cols, const float *X, int64_t ldx, in
 // The column exponents (as colexp_kernel) and the pieces scaled by them (as
 // split_kernel) in one pass per column: workgroup j finds column j's maximum,
 // writes e[j], then splits the column, whose second read comes from cache.
+// Columns of up to kRegRows rows are read once, into registers (thread t
+// owns rows 8 t + 2048 i .. + 7), for both the maximum and the pieces.
+constexpr int kRegRows = 16384;
+
 template <int PREC>
 __global__ __launch_bounds__(256) void split_exp_kernel(int rows, const float *X, int64_t ldx, int *col_exp,
                                                         uint16_t *out, int64_t ldo, int64_t stride)
 {
     const int j = blockIdx.x;
     const float *x = X + (int64_t)j * ldx;
+    const int prow_r = (rows + 63) / 64 * 64;
+    if (prow_r <= kRegRows) {
+        constexpr int NI = kRegRows / 2048;
+        __shared__ float wmax[4];
+        float v[NI][8];
+        float mx = 0.0f;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int i0 = 8 * (int)threadIdx.x + 2048 * i;
+            if (i0 < rows) load_rows8(x, rows, i0, v[i]);
+            else
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[i][e] = 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) mx = fmaxf(mx, fabsf(v[i][e]));
+#pragma unroll
+        for (int mask = 32; mask > 0; mask >>= 1) mx = fmaxf(mx, __shfl_xor(mx, mask, 64));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        mx = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        int ex = 0;
+        if (mx > 0.0f && mx < INFINITY) {
+            int fe;
+            frexpf(mx, &fe);
+            ex = 15 - fe;
+        }
+        if (threadIdx.x == 0) col_exp[j] = ex;
+        const float sc = ldexpf(1.0f, ex);
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int i0 = 8 * (int)threadIdx.x + 2048 * i;
+            if (i0 < prow_r) store_rows8<PREC>(v[i], out + (int64_t)j * ldo, rows, prow_r, stride, i0, sc);
+        }
+        return;
+    }
     const int ex = column_exp(rows, x);
     if (threadIdx.x == 0) col_exp[j] = ex;
     const float sc = ldexpf(1.0f, ex);
