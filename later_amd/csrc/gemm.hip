@@ -1364,22 +1364,31 @@ __global__ __launch_bounds__(256) void split_t_kernel(int rows, int cols, const 
     __shared__ float tile[64][65];
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // All sixteen loads (and exponents) from clamped in-bounds addresses
+    // first, so they are in flight together; the values outside the matrix
+    // (or below the diagonal) are replaced by zeros afterwards.
+    const int i = i0 + tx, ic = min(i, rows - 1);
+    float xv[16];
+    int ev[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        const int jj = ty + 4 * q, i = i0 + tx, j = j0 + jj;
-        float v = 0.0f;
-        if (i < rows && j < cols && !(tri && i > j)) {
-            float x = X[i + (int64_t)j * ldx];
-            if (tri && i == j) x -= shift;
-            v = x * ldexpf(1.0f, col_exp ? col_exp[j] : a_exp);
-        }
-        tile[jj][tx] = v;
+        const int jc = min(j0 + ty + 4 * q, cols - 1);
+        xv[q] = X[ic + (int64_t)jc * ldx];
+        ev[q] = col_exp ? col_exp[jc] : a_exp;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int jj = ty + 4 * q, j = j0 + jj;
+        float x = xv[q];
+        if (tri && i == j) x -= shift;
+        const float v = x * ldexpf(1.0f, ev[q]);
+        tile[jj][tx] = (i < rows && j < cols && !(tri && i > j)) ? v : 0.0f;
     }
     __syncthreads();
     // Thread (ii, jq): 16 consecutive j of row i0 + ii, two 16-byte stores per
     // piece (columns cols .. roundup(cols, 64) - 1 are the zero K padding).
-    const int ii = threadIdx.x >> 2, jq = (threadIdx.x & 3) * 16, i = i0 + ii;
-    if (i < rows) {
+    const int ii = threadIdx.x >> 2, jq = (threadIdx.x & 3) * 16, io = i0 + ii;
+    if (io < rows) {
         ushort8 pv[NP][2];
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
@@ -1391,7 +1400,7 @@ __global__ __launch_bounds__(256) void split_t_kernel(int rows, int cols, const 
         if (j0 + jq + 16 <= jw) {
 #pragma unroll
             for (int c = 0; c < NP; ++c) {
-                ushort8 *dst = (ushort8 *)(out + c * stride + (int64_t)i * ldo + j0 + jq);
+                ushort8 *dst = (ushort8 *)(out + c * stride + (int64_t)io * ldo + j0 + jq);
                 dst[0] = pv[c][0];
                 dst[1] = pv[c][1];
             }
@@ -1400,7 +1409,7 @@ __global__ __launch_bounds__(256) void split_t_kernel(int rows, int cols, const 
             for (int u = 0; u < 16; ++u)
                 if (j0 + jq + u < jw)
 #pragma unroll
-                    for (int c = 0; c < NP; ++c) out[c * stride + (int64_t)i * ldo + j0 + jq + u] = pv[c][u >> 3][u & 7];
+                    for (int c = 0; c < NP; ++c) out[c * stride + (int64_t)io * ldo + j0 + jq + u] = pv[c][u >> 3][u & 7];
         }
     }
 }
