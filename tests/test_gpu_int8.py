@@ -150,3 +150,23 @@ def test_int8_tall_uses_three_pieces(later):
         res.append((A, R))
     assert torch.equal(res[0][0], res[1][0]) and torch.equal(res[0][1], res[1][1])
     assert later.check_orth(res[0][0]) < 1e-5
+
+
+def test_int8_unaligned_columns(later):
+    """lda = m + 3 and m not a multiple of 64: every column starts off a
+    16-byte boundary, so the splits take their scalar paths; the mixed form
+    still meets the gates and repeats its bits."""
+    import torch
+    m, n = 3001, 1100
+    A0 = later.generate_normal(m, n, 7, 0)
+    out = []
+    for _ in range(2):
+        Ap = later.colmajor(m, n, ld=m + 3)
+        Ap.copy_(A0)
+        R = later.rgsqrf(Ap, o=later.opts("mixed"))
+        _sync()
+        out.append((Ap, R))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+    Q, R = out[0]
+    assert later.check_orth(Q) < 1e-5
+    assert later.check_backward(A0, Q, R) < 1e-6
