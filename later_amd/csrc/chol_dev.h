@@ -268,68 +268,65 @@ template <int W> __device__ __forceinline__ bool factor(double *Gs, double *Rs, 
     if (t < 64) bad = cholesky_wave<LDG, W>(Gs, d0, Rs, Ris);
     if constexpr (W == 64) {
         __syncthreads();
-        // The four 32 x 32 products below: thread t makes column j = t % 32
-        // of rows i = t / 32 + 8 u, u < 4, over the full k range (the
+        // The four 32 x 32 x 32 products below run on FP64 MFMA: wave w makes
+        // the 16 x 16 tile (w / 2, w % 2) of each, over the full k range (the
         // triangular factors' zero halves are stored, so the sums are the
-        // same), with the shared operand loaded once per k and every load
-        // independent of the running sums.
-        const int j = t & 31, i0 = t >> 5;
-        double s[4];
+        // same).  Result layout (v_mfma_f64_16x16x4f64): lane l, entry r holds
+        // tile element (l / 16 + 4 r, l % 16).
+        const int lane = t & 63, wv = t >> 6, ti = 16 * (wv >> 1), tj = 16 * (wv & 1);
+        const int li = lane & 15, lk = lane >> 4;
+        auto product = [&](auto a, auto b, doublex4 c) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                c = __builtin_amdgcn_mfma_f64_16x16x4f64(a(ti + li, 4 * s + lk), b(4 * s + lk, tj + li), c, 0, 0, 0);
+            return c;
+        };
+        const doublex4 zero = doublex4{0.0, 0.0, 0.0, 0.0};
         // R12 = R11^{-T} B -> Rs[i][32 + j]; zero lower-left blocks.
+        {
+            const doublex4 c = product([&](int i, int k) { return Ris[k * W + i]; },
+                                       [&](int k, int j) { return Gs[k * LDG + 32 + j]; }, zero);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) s[u] = 0.0;
+            for (int r = 0; r < 4; ++r) Rs[(ti + lk + 4 * r) * W + 32 + tj + li] = c[r];
+            const int j = t & 31, i0 = t >> 5;
 #pragma unroll
-        for (int k = 0; k < 32; ++k) {
-            const double b = Gs[k * LDG + 32 + j];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) s[u] = fma(Ris[k * W + i0 + 8 * u], b, s[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = i0 + 8 * u;
-            Rs[i * W + 32 + j] = s[u];
-            Rs[(32 + i) * W + j] = 0.0;
-            Ris[(32 + i) * W + j] = 0.0;
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + 8 * u;
+                Rs[(32 + i) * W + j] = 0.0;
+                Ris[(32 + i) * W + j] = 0.0;
+            }
         }
         __syncthreads();
         // S = C - R12^T R12, in place over C.
+        {
+            doublex4 c;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) s[u] = Gs[(32 + i0 + 8 * u) * LDG + 32 + j];
+            for (int r = 0; r < 4; ++r) c[r] = Gs[(32 + ti + lk + 4 * r) * LDG + 32 + tj + li];
+            c = product([&](int i, int k) { return -Rs[k * W + 32 + i]; },
+                        [&](int k, int j) { return Rs[k * W + 32 + j]; }, c);
+            __syncthreads();  // every wave has read C before any writes S over it
 #pragma unroll
-        for (int k = 0; k < 32; ++k) {
-            const double b = Rs[k * W + 32 + j];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) s[u] = fma(-Rs[k * W + 32 + i0 + 8 * u], b, s[u]);
+            for (int r = 0; r < 4; ++r) Gs[(32 + ti + lk + 4 * r) * LDG + 32 + tj + li] = c[r];
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) Gs[(32 + i0 + 8 * u) * LDG + 32 + j] = s[u];
         __syncthreads();
         if (t < 64) bad |= cholesky_wave<LDG, W>(Gs + 32 * LDG + 32, d0 + 32, Rs + 32 * W + 32, Ris + 32 * W + 32);
         __syncthreads();
         // T = R12 R22^{-1} -> over A (Gs[i][j]).
+        {
+            const doublex4 c = product([&](int i, int k) { return Rs[i * W + 32 + k]; },
+                                       [&](int k, int j) { return Ris[(32 + k) * W + 32 + j]; }, zero);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) s[u] = 0.0;
-#pragma unroll
-        for (int k = 0; k < 32; ++k) {
-            const double b = Ris[(32 + k) * W + 32 + j];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) s[u] = fma(Rs[(i0 + 8 * u) * W + 32 + k], b, s[u]);
+            for (int r = 0; r < 4; ++r) Gs[(ti + lk + 4 * r) * LDG + tj + li] = c[r];
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) Gs[(i0 + 8 * u) * LDG + j] = s[u];
         __syncthreads();
         // R^{-1}[i][32 + j] = -(R11^{-1} T)[i][j].  A bad pivot k leaves, as
         // in one block, row k of R and column k of R^{-1} zero.
+        {
+            const doublex4 c = product([&](int i, int k) { return -Ris[i * W + k]; },
+                                       [&](int k, int j) { return Gs[k * LDG + j]; }, zero);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) s[u] = 0.0;
-#pragma unroll
-        for (int k = 0; k < 32; ++k) {
-            const double b = Gs[k * LDG + j];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) s[u] = fma(Ris[(i0 + 8 * u) * W + k], b, s[u]);
+            for (int r = 0; r < 4; ++r) Ris[(ti + lk + 4 * r) * W + 32 + tj + li] = c[r];
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) Ris[(i0 + 8 * u) * W + 32 + j] = -s[u];
     }
     return bad;
 }
